@@ -1,0 +1,332 @@
+// C ABI of libbatchlayout.so: graph handles, parameter defaults, run dispatch.
+// Replaces the reference's algorithms constructor (sample/algorithms.cpp:3-15) and the
+// per-algorithm methods (sample/algorithms.h:67-80); see include/batchlayout.h.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bl_internal.h"
+
+namespace bl {
+
+int device_error(hipError_t e, const char *what, int line) {
+    fprintf(stderr, "batchlayout: HIP error '%s' (%d) in %s [line %d]\n", hipGetErrorString(e), (int)e, what, line);
+    return BL_E_DEVICE;
+}
+
+int DevBuf::ensure(size_t want) {
+    if (want <= bytes && p) return 0;
+    release();
+    if (want == 0) return 0;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        return BL_E_NOMEM;
+    }
+    bytes = want;
+    return 0;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+void BHWork::release() {
+    DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &hist, &spos, &frame, &isrep, &nodeid, &bnd_lo,
+                     &node_child, &node_depth, &node_cnt, &node_parent, &node_cxy, &node_rep, &scan_tmp,
+                     &rep, &misc};
+    for (DevBuf *b : all) b->release();
+    n_alloc = 0;
+}
+
+// The code objects are built for gfx950 only: refuse anything else up front.
+static int check_device_once() {
+    static std::once_flag once;
+    static int status = BL_E_DEVICE;
+    std::call_once(once, [] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+            fprintf(stderr, "batchlayout: no HIP device visible\n");
+            return;
+        }
+        hipDeviceProp_t prop;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            fprintf(stderr, "batchlayout: device %d is %s; this build targets gfx950 only\n", dev, prop.gcnArchName);
+            return;
+        }
+        status = BL_OK;
+    });
+    return status;
+}
+
+}  // namespace bl
+
+using namespace bl;
+
+extern "C" int bl_params_default(bl_params *p) {
+    if (!p) return BL_E_ARG;
+    memset(p, 0, sizeof(*p));
+    p->algo = BL_ALGO_CACHE;
+    p->iters = 600;
+    p->batch = 256;
+    p->converged = 0;
+    p->bht = 1.2;
+    p->engt = 0.01;
+    p->weight = 1.0;
+    p->step0 = 1.0;
+    p->precision = BL_PREC_FP32;
+    return BL_OK;
+}
+
+extern "C" int bl_graph_create(int32_t n, int64_t nnz, const int32_t *rowptr, const int32_t *colids,
+                               const float *w_edge, bl_graph **out) {
+    if (!out) return BL_E_ARG;
+    *out = nullptr;
+    if (n <= 0 || nnz < 0 || !rowptr || (nnz > 0 && !colids)) return BL_E_ARG;
+    if (nnz > INT32_MAX) return BL_E_ARG;
+    // CSR validation (the reference trusts its own builder; the ABI does not)
+    if (rowptr[0] != 0 || rowptr[n] != nnz) return BL_E_GRAPH;
+    for (int32_t i = 0; i < n; ++i)
+        if (rowptr[i + 1] < rowptr[i]) return BL_E_GRAPH;
+    for (int64_t k = 0; k < nnz; ++k)
+        if (colids[k] < 0 || colids[k] >= n) return BL_E_GRAPH;
+    int rc = check_device_once();
+    if (rc) return rc;
+
+    // Neighbours the merge pointer of algorithms.cpp:758-762 meets: the pointer starts at
+    // rowptr[i], matches when j == colids[ptr] and advances only while ptr < rowptr[i+1]-1.
+    // With j ascending that is the longest strictly increasing prefix of the row; an empty
+    // row keeps the pointer on the next row's first entry, which therefore acts as one
+    // neighbour (an empty last row would read past the array: no neighbour here).
+    std::vector<int32_t> erp(n + 1, 0), eci;
+    std::vector<float> ew;
+    eci.reserve((size_t)nnz);
+    int32_t maxdeg = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t s = rowptr[i], e = rowptr[i + 1];
+        int32_t cnt = 0;
+        if (s < e) {
+            int32_t last = colids[s];
+            eci.push_back(last);
+            if (w_edge) ew.push_back(w_edge[s]);
+            ++cnt;
+            for (int64_t k = s + 1; k < e && colids[k] > last; ++k) {
+                last = colids[k];
+                eci.push_back(last);
+                if (w_edge) ew.push_back(w_edge[k]);
+                ++cnt;
+            }
+        } else if (s < nnz) {
+            eci.push_back(colids[s]);
+            if (w_edge) ew.push_back(w_edge[s]);
+            ++cnt;
+        }
+        erp[i + 1] = erp[i] + cnt;
+        if (cnt > maxdeg) maxdeg = cnt;
+    }
+
+    bl_graph *g = new (std::nothrow) bl_graph();
+    if (!g) return BL_E_NOMEM;
+    g->n = n;
+    g->nnz = nnz;
+    g->has_w = w_edge != nullptr;
+    g->eff_max_deg = maxdeg;
+    auto fail = [&](int code) {
+        bl_graph_destroy(g);
+        return code;
+    };
+    if (hipGetDevice(&g->device) != hipSuccess) return fail(BL_E_DEVICE);
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) return fail(BL_E_DEVICE);
+    const size_t nnzb = (size_t)(nnz > 0 ? nnz : 1);
+    const size_t effb = eci.empty() ? 1 : eci.size();
+    if (g->rowptr.ensure(sizeof(int32_t) * (n + 1)) || g->colids.ensure(sizeof(int32_t) * nnzb) ||
+        g->eff_rowptr.ensure(sizeof(int32_t) * (n + 1)) || g->eff_colids.ensure(sizeof(int32_t) * effb))
+        return fail(BL_E_NOMEM);
+    if (w_edge && (g->w_edge.ensure(sizeof(float) * nnzb) || g->eff_w.ensure(sizeof(float) * effb)))
+        return fail(BL_E_NOMEM);
+    hipStream_t s = g->stream;
+    if (hipMemcpyAsync(g->rowptr.p, rowptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+        (nnz > 0 && hipMemcpyAsync(g->colids.p, colids, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s) != hipSuccess) ||
+        hipMemcpyAsync(g->eff_rowptr.p, erp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+        (!eci.empty() && hipMemcpyAsync(g->eff_colids.p, eci.data(), sizeof(int32_t) * eci.size(),
+                                        hipMemcpyHostToDevice, s) != hipSuccess))
+        return fail(BL_E_DEVICE);
+    if (w_edge && ((nnz > 0 && hipMemcpyAsync(g->w_edge.p, w_edge, sizeof(float) * nnz, hipMemcpyHostToDevice, s) !=
+                                   hipSuccess) ||
+                   (!ew.empty() &&
+                    hipMemcpyAsync(g->eff_w.p, ew.data(), sizeof(float) * ew.size(), hipMemcpyHostToDevice, s) !=
+                        hipSuccess)))
+        return fail(BL_E_DEVICE);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(BL_E_DEVICE);
+    *out = g;
+    return BL_OK;
+}
+
+extern "C" int bl_graph_destroy(bl_graph *g) {
+    if (!g) return BL_E_ARG;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
+                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch};
+    for (DevBuf *b : all) b->release();
+    g->bh.release();
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return BL_OK;
+}
+
+extern "C" int bl_graph_size(const bl_graph *g, int32_t *n, int64_t *nnz) {
+    if (!g) return BL_E_ARG;
+    if (n) *n = g->n;
+    if (nnz) *nnz = g->nnz;
+    return BL_OK;
+}
+
+static int validate_params(const bl_params *p) {
+    if (!p) return BL_E_ARG;
+    if (p->batch <= 0 || p->iters < 0) return BL_E_ARG;
+    if (p->precision != BL_PREC_FP32 && p->precision != BL_PREC_FP64) return BL_E_ARG;
+    switch (p->algo) {
+        case BL_ALGO_CACHE: case BL_ALGO_LINLOG: case BL_ALGO_BH: case BL_ALGO_BHCACHE: case BL_ALGO_FA:
+            return BL_OK;
+        default:
+            return BL_E_ARG;
+    }
+}
+
+// Runs one layout on device-resident positions.  Dispatch mirrors Test/BatchLayout.cpp:145-170.
+static int run_on_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats *st) {
+    bl_stats local;
+    memset(&local, 0, sizeof(local));
+    int rc = BL_OK;
+    switch (p->algo) {
+        case BL_ALGO_CACHE:
+            rc = run_minibatch(g, d_xy, p, LAW_FR, 0.999, p->step0, p->iters, p->converged != 0, &local);
+            break;
+        case BL_ALGO_LINLOG:
+            rc = run_minibatch(g, d_xy, p, LAW_LINLOG, 0.99, p->step0, p->iters, false, &local);
+            break;
+        case BL_ALGO_FA:
+            rc = run_minibatch(g, d_xy, p, LAW_FA, 0.99, p->step0, p->iters, false, &local);
+            break;
+        case BL_ALGO_BH:
+            rc = run_bh(g, d_xy, p, p->step0, p->iters, p->bht, &local);
+            break;
+        case BL_ALGO_BHCACHE: {
+            // approxCacheBlockBH (algorithms.cpp:1192-1215): (int)(0.8*iters) BH iterations at
+            // theta 1.2 (the -bht flag is ignored), then the rest as algo 2 starting from
+            // STEP = pow(0.999, 4*rest) (the flag == 1 branch, :736-737).
+            const int approx = (int)(p->iters * 0.8);
+            rc = run_bh(g, d_xy, p, p->step0, approx, 1.2, &local);
+            if (rc == BL_OK) {
+                const int rest = p->iters - approx;
+                rc = run_minibatch(g, d_xy, p, LAW_FR, 0.999, pow(0.999, 4 * rest), rest, false, &local);
+                local.iters_done = p->iters;
+            }
+            break;
+        }
+        default:
+            rc = BL_E_ARG;
+    }
+    local.status = rc;
+    if (st) *st = local;
+    return rc;
+}
+
+extern "C" int bl_run_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats *st, void *stream) {
+    if (!g || !d_xy) return BL_E_ARG;
+    int rc = validate_params(p);
+    if (rc) return rc;
+    hipStream_t own = g->stream;
+    if (stream) g->stream = static_cast<hipStream_t>(stream);
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = run_on_device(g, d_xy, p, st);
+    if (hipStreamSynchronize(g->stream) != hipSuccess && rc == BL_OK) rc = BL_E_DEVICE;
+    const auto t1 = std::chrono::steady_clock::now();
+    g->stream = own;
+    if (st) {
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->status = rc;
+    }
+    return rc;
+}
+
+extern "C" int bl_run(bl_graph *g, double *xy_inout, const bl_params *p, bl_stats *st) {
+    if (!g || !xy_inout) return BL_E_ARG;
+    int rc = validate_params(p);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * 2 * (size_t)g->n;
+    if (g->pos.ensure(bytes)) return BL_E_NOMEM;
+    BL_HIP(hipMemcpyAsync(g->pos.p, xy_inout, bytes, hipMemcpyHostToDevice, g->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = run_on_device(g, g->pos.as<double>(), p, st);
+    if (hipStreamSynchronize(g->stream) != hipSuccess && rc == BL_OK) rc = BL_E_DEVICE;
+    const auto t1 = std::chrono::steady_clock::now();
+    if (rc == BL_OK) {
+        BL_HIP(hipMemcpyAsync(xy_inout, g->pos.p, bytes, hipMemcpyDeviceToHost, g->stream));
+        BL_HIP(hipStreamSynchronize(g->stream));
+    }
+    if (st) {
+        st->seconds = std::chrono::duration<double>(t1 - t0).count();
+        st->status = rc;
+    }
+    return rc;
+}
+
+extern "C" int bl_bh_morton(int32_t n, const double *xy, uint32_t *xs, uint32_t *ys, double *maxmin) {
+    if (n <= 0 || !xy || !xs || !ys) return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return bh_morton_host(n, xy, xs, ys, maxmin);
+}
+
+extern "C" int bl_bh_tree(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, int32_t *nnodes,
+                          int32_t *root, double *diam, double *cx, double *cy, int32_t *child) {
+    if (n <= 0 || !xy || !perm || !nnodes || !root) return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return bh_tree_host(n, xy, perm, max_nodes, nnodes, root, diam, cx, cy, child);
+}
+
+extern "C" int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t nq, const double *q, double *out,
+                            int32_t precision) {
+    if (n <= 0 || !xy || nq < 0 || (nq > 0 && (!q || !out))) return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return bh_forces_host(n, xy, theta, nq, q, out, precision);
+}
+
+extern "C" int bl_run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,
+                              const int32_t *colids, double *xy_inout, const bl_params *p, double *energies,
+                              bl_stats *st) {
+    if (count <= 0 || !voff || !eoff || !rowptr || !xy_inout || !p) return BL_E_ARG;
+    int rc = validate_params(p);
+    if (rc) return rc;
+    if (p->algo != BL_ALGO_CACHE && p->algo != BL_ALGO_FA) return BL_E_ARG;
+    rc = check_device_once();
+    if (rc) return rc;
+    return run_batched(count, voff, eoff, rowptr, colids, xy_inout, p, energies, st);
+}
+
+extern "C" int bl_device_info(char *buf, int32_t len) {
+    if (!buf || len <= 0) return BL_E_ARG;
+    int dev = 0, count = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || hipGetDevice(&dev) != hipSuccess ||
+        hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        snprintf(buf, (size_t)len, "no HIP device");
+        return BL_E_DEVICE;
+    }
+    snprintf(buf, (size_t)len, "%s %s CUs=%d clock=%dkHz mem=%zuMB", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, prop.clockRate, (size_t)(prop.totalGlobalMem >> 20));
+    return BL_OK;
+}

@@ -1,0 +1,908 @@
+// Barnes-Hut path of BatchLayout (-algo 4) on CDNA4 (gfx950).
+//
+// Reference: sample/algorithms.cpp:1020-1083 (the layout loop), sample/MortonCode.h:42-97
+// (bounding box, 32-bit scaling, y-over-x Morton order), sample/BarnesHut.h:114-220 (sort
+// of the coordinates and recursive quadtree build, unweighted centroids) and :69-86
+// (traversal: an accepted cell or a leaf contributes a unit vector).
+//
+// Per iteration, from the positions at the start of the iteration (:1041-1047):
+//   1. k_frame_*   min x, min y, MAXMIN = max(dy, dx)                       (FP64, exact)
+//   2. k_keys      64-bit key = interleave(yscale, xscale), y bit above x bit
+//   3. radix sort  LSD, 8 passes of 8 bits over (key, vertex id), stable
+//   4. k_gather    positions in Morton order
+//   5. tree        node = cell where two adjacent keys first differ (base-4 digit d of
+//                  k_i ^ k_i+1); its range, quadrants and children come from binary
+//                  searches on the sorted keys -- the reference's tree with single-child
+//                  chains collapsed into their lowest node (a chain shares one centroid,
+//                  and the smallest diameter in the chain decides acceptance, so the
+//                  traversal result is the same)
+//   6. k_centroid  unweighted mean of the non-empty quadrants' centroids, bottom-up,
+//                  summed in quadrant order as :197-217
+//   7. k_rep*      repulsion of every vertex, one wave per 64 Morton-consecutive vertices
+//                  with a shared stack of (node, lane mask): the vertex's own position is
+//                  the iteration-start one (it only moves in its own batch step), so all
+//                  n traversals run in one launch
+//   8. k_attract_move  one workgroup walks the ceil(n/B) batches in order: attraction
+//                  over every CSR entry with live positions (:1055-1058), minus the
+//                  repulsion, then the unit-step move (:1062-1066)
+//   9. energy      sum of |F_i|^2
+//
+// FP32 mode evaluates the opening test D/|c-v| < theta in FP32 with a relative guard
+// band and settles the borderline cases in FP64, so the set of accepted cells is the
+// FP64 one; the unit vectors are summed in FP32.  FP64 mode visits cells in the
+// reference's depth-first quadrant order and uses its FP64 operations, so the force is
+// the reference's bit for bit.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bl_internal.h"
+
+namespace bl {
+
+int launch_energy(const double *eterm, int n, double *out, int precision, hipStream_t s);
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int SORT_ITEMS = 16;                 // keys per thread per sort tile
+constexpr int SORT_TILE = WG * SORT_ITEMS;     // 4096 keys per workgroup
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = WG * SCAN_ITEMS;     // 2048 entries per workgroup
+constexpr int STACK = 128;                     // per-wave traversal stack (depth <= 32 -> <= 100 entries)
+
+struct BHNode {
+    double2 c;     // centroid
+    int4 ch;       // children: >= 0 node, -1 empty, <= -2 leaf at Morton rank (-c-2)
+    int depth;     // base-4 level of the cell: diameter = MAXMIN * 2^-depth
+    int nint;      // number of internal children
+    int parent;
+    int cnt;       // bottom-up arrival counter
+};
+
+// status word bits
+constexpr int ST_DUPKEY = 1;
+
+// ---------------------------------------------------------------------------------------
+// 1. bounding box
+__global__ __launch_bounds__(WG) void k_frame_partial(const double2 *__restrict__ pos, int n,
+                                                      double4 *__restrict__ part) {
+    __shared__ double4 s[WG];
+    double4 a = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);  // minx maxx miny maxy
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
+        const double2 p = pos[i];
+        a.x = fmin(a.x, p.x);
+        a.y = fmax(a.y, p.x);
+        a.z = fmin(a.z, p.y);
+        a.w = fmax(a.w, p.y);
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double4 b = s[threadIdx.x + w];
+            double4 &c = s[threadIdx.x];
+            c.x = fmin(c.x, b.x);
+            c.y = fmax(c.y, b.y);
+            c.z = fmin(c.z, b.z);
+            c.w = fmax(c.w, b.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// frame[0] = minX, frame[1] = minY, frame[2] = MAXMIN = max(maxY-minY, maxX-minX) (:45-51)
+__global__ __launch_bounds__(WG) void k_frame_final(const double4 *__restrict__ part, int np, double *frame) {
+    __shared__ double4 s[WG];
+    double4 a = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int i = threadIdx.x; i < np; i += WG) {
+        const double4 b = part[i];
+        a.x = fmin(a.x, b.x);
+        a.y = fmax(a.y, b.y);
+        a.z = fmin(a.z, b.z);
+        a.w = fmax(a.w, b.w);
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const double4 b = s[threadIdx.x + w];
+            double4 &c = s[threadIdx.x];
+            c.x = fmin(c.x, b.x);
+            c.y = fmax(c.y, b.y);
+            c.z = fmin(c.z, b.z);
+            c.w = fmax(c.w, b.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double dy = s[0].w - s[0].z, dx = s[0].y - s[0].x;
+        frame[0] = s[0].x;
+        frame[1] = s[0].z;
+        frame[2] = (dy < dx) ? dx : dy;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// 2. Morton keys.  xscale(x) = (uint32_t)(0xffffffff * (x - minX) / MAXMIN), MortonCode.h:62-67.
+__device__ __forceinline__ uint32_t scale32(double v, double mn, double maxmin) {
+    double t = 4294967295.0 * (v - mn);
+    t = t / maxmin;
+    return (uint32_t)t;
+}
+
+__device__ __forceinline__ uint64_t spread(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x << 2)) & 0x3333333333333333ull;
+    x = (x | (x << 1)) & 0x5555555555555555ull;
+    return x;
+}
+
+__global__ __launch_bounds__(WG) void k_keys(const double2 *__restrict__ pos, int n, const double *__restrict__ frame,
+                                             uint64_t *__restrict__ keys, int *__restrict__ ids,
+                                             uint32_t *__restrict__ xs, uint32_t *__restrict__ ys) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    const double mnx = frame[0], mny = frame[1], mm = frame[2];
+    const double2 p = pos[i];
+    const uint32_t sx = scale32(p.x, mnx, mm), sy = scale32(p.y, mny, mm);
+    keys[i] = (spread(sy) << 1) | spread(sx);
+    ids[i] = i;
+    if (xs) {
+        xs[i] = sx;
+        ys[i] = sy;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// generic exclusive scan of int32 (3 phases)
+__device__ __forceinline__ int block_excl_scan(int v, int *lds, int &total) {
+    // lds: WG ints
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < WG; off <<= 1) {
+        const int add = tid >= off ? lds[tid - off] : 0;
+        __syncthreads();
+        lds[tid] += add;
+        __syncthreads();
+    }
+    total = lds[WG - 1];
+    const int incl = lds[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_sums(const int *__restrict__ in, int n, int *__restrict__ bsum) {
+    __shared__ int lds[WG];
+    const int base = blockIdx.x * SCAN_TILE;
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int i = base + k * WG + threadIdx.x;
+        if (i < n) a += in[i];
+    }
+    int total;
+    (void)block_excl_scan(a, lds, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// single workgroup: exclusive scan of the block sums in place, grand total at bsum[nb]
+__global__ __launch_bounds__(WG) void k_scan_top(int *__restrict__ bsum, int nb) {
+    __shared__ int lds[WG];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += WG) {
+        const int i = b0 + threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        int total;
+        const int ex = block_excl_scan(v, lds, total);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_apply(const int *__restrict__ in, int n, const int *__restrict__ bsum,
+                                                   int *__restrict__ out) {
+    __shared__ int lds[WG];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;  // contiguous per thread
+    int v[SCAN_ITEMS];
+    int a = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        a += v[k];
+    }
+    int total;
+    int run = block_excl_scan(a, lds, total) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// 3. LSD radix sort, 8-bit digits.  hist is digit-major: hist[d * nblk + blk].
+__global__ __launch_bounds__(WG) void k_sort_hist(const uint64_t *__restrict__ keys, int n, int shift, int nblk,
+                                                  int *__restrict__ hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * SORT_TILE;
+#pragma unroll 4
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+        const int i = base + r * WG + threadIdx.x;
+        if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255u)], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items are ranked in tile order (round r, then thread index), using wave
+// ballots to find the lanes that share a digit and per-wave counts in LDS.
+__global__ __launch_bounds__(WG) void k_sort_scatter(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
+                                                     uint64_t *__restrict__ kout, int *__restrict__ vout, int n,
+                                                     int shift, int nblk, const int *__restrict__ offs) {
+    __shared__ int running[256];
+    __shared__ int wcount[WG / 64][256];
+    __shared__ int base_off[256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    running[tid] = 0;
+    base_off[tid] = offs[tid * nblk + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ITEMS; ++r) {
+#pragma unroll
+        for (int q = 0; q < WG / 64; ++q) wcount[q][tid] = 0;
+        __syncthreads();
+        const int i = base + r * WG + tid;
+        const bool valid = i < n;
+        const uint64_t k = valid ? kin[i] : 0ull;
+        const int v = valid ? vin[i] : 0;
+        const int d = (int)((k >> shift) & 255u);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wcount[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pos = base_off[d] + running[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wcount[q][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int q = 0; q < WG / 64; ++q) add += wcount[q][tid];
+        running[tid] += add;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// 4. gather positions into Morton order
+__global__ __launch_bounds__(WG) void k_gather(const double2 *__restrict__ pos, const int *__restrict__ perm, int n,
+                                               double2 *__restrict__ spos) {
+    const int k = blockIdx.x * WG + threadIdx.x;
+    if (k < n) spos[k] = pos[perm[k]];
+}
+
+// ---------------------------------------------------------------------------------------
+// 5. tree
+__device__ __forceinline__ int digit_at(uint64_t k, int d) { return (int)((k >> (62 - 2 * d)) & 3u); }
+__device__ __forceinline__ uint64_t prefix_of(uint64_t k, int d) { return d == 0 ? 0ull : (k >> (64 - 2 * d)); }
+
+// first index in [lo, hi) whose 2d-bit prefix equals pfx (keys sorted)
+__device__ __forceinline__ int first_with_prefix(const uint64_t *keys, int lo, int hi, int d, uint64_t pfx) {
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (prefix_of(keys[m], d) < pfx) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+// last index in [lo, hi) whose prefix equals pfx, given keys[lo] has it
+__device__ __forceinline__ int last_with_prefix(const uint64_t *keys, int lo, int hi, int d, uint64_t pfx) {
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (prefix_of(keys[m], d) <= pfx) lo = m; else hi = m;
+    }
+    return lo;
+}
+// first index in [lo, hi) whose digit d is >= q (lower_bound of BarnesHut.h:143-146)
+__device__ __forceinline__ int lower_digit(const uint64_t *keys, int lo, int hi, int d, int q) {
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (digit_at(keys[m], d) < q) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+// the node whose points are [s, e] (inclusive, e > s): the boundary where its first
+// quadrant ends
+__device__ __forceinline__ int rep_of_range(const uint64_t *keys, int s, int e) {
+    const int d = __clzll((long long)(keys[s] ^ keys[e])) >> 1;
+    const int q0 = digit_at(keys[s], d);
+    return lower_digit(keys, s, e + 1, d, q0 + 1) - 1;
+}
+
+// boundary i (between Morton ranks i and i+1) is a node representative when it ends the
+// first quadrant of the cell where k_i and k_i+1 split
+__global__ __launch_bounds__(WG) void k_tree_bounds(const uint64_t *__restrict__ keys, int n, int *__restrict__ isrep,
+                                                    int *__restrict__ status) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n - 1) return;
+    const uint64_t a = keys[i], b = keys[i + 1];
+    if (a == b) {
+        atomicOr(status, ST_DUPKEY);
+        isrep[i] = 0;
+        return;
+    }
+    const int d = __clzll((long long)(a ^ b)) >> 1;
+    const uint64_t pfx = prefix_of(a, d);
+    const int L = d == 0 ? 0 : first_with_prefix(keys, 0, i + 1, d, pfx);
+    isrep[i] = digit_at(keys[L], d) == digit_at(a, d) ? 1 : 0;
+}
+
+// root = the node spanning all points; also the node count (scan total)
+__global__ void k_tree_root(const uint64_t *__restrict__ keys, int n, const int *__restrict__ nodeid,
+                            const int *__restrict__ scan_total, int *__restrict__ misc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || misc[0]) return;
+    misc[1] = nodeid[rep_of_range(keys, 0, n - 1)];
+    misc[2] = *scan_total;
+}
+
+__global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ keys, int n,
+                                                   const int *__restrict__ isrep, const int *__restrict__ nodeid,
+                                                   BHNode *__restrict__ nodes, int *__restrict__ misc,
+                                                   const int *__restrict__ status) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (*status) return;
+    if (i >= n - 1 || !isrep[i]) return;
+    const int v = nodeid[i];
+    const uint64_t a = keys[i];
+    const int d = __clzll((long long)(a ^ keys[i + 1])) >> 1;
+    const uint64_t pfx = prefix_of(a, d);
+    const int L = d == 0 ? 0 : first_with_prefix(keys, 0, i + 1, d, pfx);
+    const int R = d == 0 ? n - 1 : last_with_prefix(keys, i + 1, n, d, pfx);
+    int bnd[5];
+    bnd[0] = L;
+    bnd[4] = R + 1;
+    for (int q = 1; q < 4; ++q) bnd[q] = lower_digit(keys, L, R + 1, d, q);
+    int ch[4];
+    int nint = 0;
+    for (int q = 0; q < 4; ++q) {
+        const int s = bnd[q], e = bnd[q + 1];
+        if (e - s == 0) {
+            ch[q] = -1;
+        } else if (e - s == 1) {
+            ch[q] = -(s + 2);
+        } else {
+            const int c = nodeid[rep_of_range(keys, s, e - 1)];
+            ch[q] = c;
+            nodes[c].parent = v;
+            ++nint;
+        }
+    }
+    nodes[v].ch = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    nodes[v].depth = d;
+    nodes[v].nint = nint;
+    nodes[v].cnt = 0;
+    if (v == misc[1]) nodes[v].parent = -1;
+}
+
+// 6. centroids, bottom-up; the last child to finish computes its parent
+__device__ __forceinline__ void centroid_of(BHNode *nodes, const double2 *spos, int v) {
+    const int4 ch = nodes[v].ch;
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    double sx = 0.0, sy = 0.0, count = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (c[q] >= 0) {
+            const double2 cc = nodes[c[q]].c;
+            sx += cc.x;
+            sy += cc.y;
+            count++;
+        } else if (c[q] <= -2) {
+            const double2 p = spos[-c[q] - 2];
+            sx += p.x;
+            sy += p.y;
+            count++;
+        }
+    }
+    if (count > 0) {
+        sx /= count;
+        sy /= count;
+    }
+    nodes[v].c = make_double2(sx, sy);
+}
+
+__global__ __launch_bounds__(WG) void k_centroid(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                 const int *__restrict__ misc) {
+    int v = blockIdx.x * WG + threadIdx.x;
+    if (misc[0] || v >= misc[2]) return;
+    if (nodes[v].nint != 0) return;
+    for (;;) {
+        centroid_of(nodes, spos, v);
+        const int u = nodes[v].parent;
+        if (u < 0) break;
+        const int old = __hip_atomic_fetch_add(&nodes[u].cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old != nodes[u].nint - 1) break;
+        v = u;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// 7. repulsion by traversal
+struct Frame {
+    double maxmin;
+    double theta;
+};
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// FP32 mode.  One wave per 64 Morton-consecutive vertices; shared stack of (node, mask).
+__global__ __launch_bounds__(WG) void k_rep32(const double2 *__restrict__ qpos, const int *__restrict__ qid, int n,
+                                              const double2 *__restrict__ spos,
+                                              const BHNode *__restrict__ nodes, const int *__restrict__ misc,
+                                              const double *__restrict__ frame, double theta,
+                                              double2 *__restrict__ rep, const int *__restrict__ status) {
+    __shared__ int stk_node[WG / 64][STACK];
+    __shared__ uint64_t stk_mask[WG / 64][STACK];
+    if (*status) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = (blockIdx.x * (WG / 64) + w) * 64 + lane;
+    const bool valid = k < n;
+    const double2 p = qpos[valid ? k : n - 1];
+    const double maxmin = frame[2];
+    const float th2 = (float)(theta * theta);
+    const float lo_band = 1.0f - 1.0f / 4096.0f, hi_band = 1.0f + 1.0f / 4096.0f;
+    float fx = 0.f, fy = 0.f;
+    const int root = misc[1];
+    int sp = 0;
+    uint64_t m0 = __ballot(valid);
+    if (m0 == 0) return;
+    if (root < 0) {  // single vertex: no other point, no force
+        if (valid) rep[qid[k]] = make_double2(0.0, 0.0);
+        return;
+    }
+    if (lane == 0) {
+        stk_node[w][0] = root;
+        stk_mask[w][0] = m0;
+    }
+    sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        --sp;
+        const int v = uni(stk_node[w][sp]);
+        const uint64_t m = stk_mask[w][sp];
+        __builtin_amdgcn_wave_barrier();
+        const bool act = (m >> lane) & 1ull;
+        const BHNode nd = nodes[v];
+        const double dx64 = nd.c.x - p.x, dy64 = nd.c.y - p.y;
+        const float dx = (float)dx64, dy = (float)dy64;
+        const float r2 = fmaf(dy, dy, dx * dx);
+        const double D = ldexp(maxmin, -nd.depth);
+        const float D32 = (float)D;
+        const float a = D32 * D32, b = th2 * r2;
+        bool acc;
+        if (a < b * lo_band) acc = true;
+        else if (a > b * hi_band) acc = false;
+        else acc = D / sqrt(dx64 * dx64 + dy64 * dy64) < theta;  // exact FP64 test, :71-72
+        if (act && acc) {
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            fx = fmaf(dx, inv, fx);
+            fy = fmaf(dy, inv, fy);
+        }
+        const uint64_t rej = __ballot(act && !acc);
+        if (rej) {
+            const int c[4] = {nd.ch.x, nd.ch.y, nd.ch.z, nd.ch.w};
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+                if (c[q] >= 0) {
+                    if (lane == 0) {
+                        stk_node[w][sp] = c[q];
+                        stk_mask[w][sp] = rej;
+                    }
+                    ++sp;
+                } else if (c[q] <= -2) {
+                    if ((rej >> lane) & 1ull) {  // leaf, :78-83
+                        const double2 l = spos[-c[q] - 2];
+                        const float lx = (float)(l.x - p.x), ly = (float)(l.y - p.y);
+                        const float lr2 = fmaf(ly, ly, lx * lx);
+                        if (lr2 > 0.f) {
+                            const float inv = __builtin_amdgcn_rsqf(lr2);
+                            fx = fmaf(lx, inv, fx);
+                            fy = fmaf(ly, inv, fy);
+                        }
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (valid) rep[qid[k]] = make_double2((double)fx, (double)fy);
+}
+
+// FP64 mode: the reference's depth-first order (quadrants 0..3, leaves in place) and its
+// FP64 operations; leaves are stack entries too so that every lane sums in its own order.
+__global__ __launch_bounds__(WG) void k_rep64(const double2 *__restrict__ qpos, const int *__restrict__ qid, int n,
+                                              const double2 *__restrict__ spos,
+                                              const BHNode *__restrict__ nodes, const int *__restrict__ misc,
+                                              const double *__restrict__ frame, double theta,
+                                              double2 *__restrict__ rep, const int *__restrict__ status) {
+    __shared__ int stk_node[WG / 64][STACK];
+    __shared__ uint64_t stk_mask[WG / 64][STACK];
+    if (*status) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = (blockIdx.x * (WG / 64) + w) * 64 + lane;
+    const bool valid = k < n;
+    const double2 p = qpos[valid ? k : n - 1];
+    const double maxmin = frame[2];
+    double fx = 0.0, fy = 0.0;
+    const int root = misc[1];
+    uint64_t m0 = __ballot(valid);
+    if (m0 == 0) return;
+    if (root < 0) {
+        if (valid) rep[qid[k]] = make_double2(0.0, 0.0);
+        return;
+    }
+    if (lane == 0) {
+        stk_node[w][0] = root;
+        stk_mask[w][0] = m0;
+    }
+    int sp = 1;
+    __builtin_amdgcn_wave_barrier();
+    while (sp > 0) {
+        --sp;
+        const int v = uni(stk_node[w][sp]);
+        const uint64_t m = stk_mask[w][sp];
+        __builtin_amdgcn_wave_barrier();
+        const bool act = (m >> lane) & 1ull;
+        if (v <= -2) {  // leaf entry
+            if (act) {
+                const double2 l = spos[-v - 2];
+                const double lx = l.x - p.x, ly = l.y - p.y;
+                const double den = sqrt(lx * lx + ly * ly);
+                if (den > 0) {
+                    const double inv = 1.0 / den;
+                    fx = fx + lx * inv;
+                    fy = fy + ly * inv;
+                }
+            }
+            continue;
+        }
+        const BHNode nd = nodes[v];
+        const double dx = nd.c.x - p.x, dy = nd.c.y - p.y;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const double D = ldexp(maxmin, -nd.depth);
+        const bool acc = D / dist < theta;
+        if (act && acc) {
+            const double inv = 1.0 / dist;
+            fx = fx + dx * inv;
+            fy = fy + dy * inv;
+        }
+        const uint64_t rej = __ballot(act && !acc);
+        if (rej) {
+            const int c[4] = {nd.ch.x, nd.ch.y, nd.ch.z, nd.ch.w};
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+                if (c[q] != -1) {
+                    if (lane == 0) {
+                        stk_node[w][sp] = c[q];
+                        stk_mask[w][sp] = rej;
+                    }
+                    ++sp;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (valid) rep[qid[k]] = make_double2(fx, fy);
+}
+
+// ---------------------------------------------------------------------------------------
+// 8. batch steps: attraction with live positions minus the precomputed repulsion, move.
+// One workgroup walks all batches of the iteration in order (Gauss-Seidel across batches,
+// Jacobi inside a batch: every force of the batch is formed before any move).
+__global__ __launch_bounds__(1024) void k_attract_move(const int *__restrict__ rowptr, const int *__restrict__ colids,
+                                                       const float *__restrict__ w_edge, double W,
+                                                       double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                       double2 *__restrict__ fbuf, int n, int B, double step,
+                                                       double *__restrict__ eterm, const int *__restrict__ status) {
+    if (*status) return;
+    const int nb = (n + B - 1) / B;
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = b * B;
+        const int b1 = min(n, b0 + B);
+        for (int i = b0 + (int)threadIdx.x; i < b1; i += 1024) {
+            const double2 pi = pos[i];
+            double fx = 0.0, fy = 0.0;
+            const int e = rowptr[i + 1];
+            for (int k = rowptr[i]; k < e; ++k) {  // every CSR entry, :1055-1058
+                const double2 pj = pos[colids[k]];
+                const double dx = pj.x - pi.x, dy = pj.y - pi.y;
+                const double Wk = w_edge ? W * (double)w_edge[k] : W;
+                const double s = Wk * sqrt(dx * dx + dy * dy);
+                fx = fx + dx * s;
+                fy = fy + dy * s;
+            }
+            const double2 r = rep[i];
+            fbuf[i] = make_double2(fx - r.x, fy - r.y);  // f - calcRepForce(x_i), :1059
+        }
+        __syncthreads();
+        for (int i = b0 + (int)threadIdx.x; i < b1; i += 1024) {
+            const double2 f = fbuf[i];
+            const double mag = sqrt(f.x * f.x + f.y * f.y);
+            double2 q = pos[i];
+            q.x = q.x + (f.x / mag) * step;
+            q.y = q.y + (f.y / mag) * step;
+            pos[i] = q;
+            eterm[i] = f.x * f.x + f.y * f.y;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host orchestration
+struct TreeBufs {
+    uint64_t *keys;  // sorted keys
+    int *perm;       // sorted vertex ids
+    double2 *spos;
+    BHNode *nodes;
+    int *misc;       // misc[0] status, misc[1] root, misc[2] node count
+    double *frame;
+};
+
+int scan_excl(const int *in, int n, int *out, int *bsum, hipStream_t s) {
+    const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(WG), 0, s, in, n, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(WG), 0, s, bsum, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(WG), 0, s, in, n, bsum, out);
+    BL_HIP(hipGetLastError());
+    return BL_OK;
+}
+
+int ensure_bh(BHWork &w, int n) {
+    if (w.n_alloc >= n) return BL_OK;
+    const size_t N = (size_t)n;
+    const int nblk = (n + SORT_TILE - 1) / SORT_TILE;
+    const int nscan = std::max(256 * nblk, n);
+    const int nsb = (nscan + SCAN_TILE - 1) / SCAN_TILE + 1;
+    if (w.keys_a.ensure(8 * N) || w.keys_b.ensure(8 * N) || w.idx_a.ensure(4 * N) || w.idx_b.ensure(4 * N) ||
+        w.hist.ensure(4 * (size_t)256 * nblk) || w.scan_tmp.ensure(4 * (size_t)(256 * nblk + 1)) ||
+        w.spos.ensure(16 * N) || w.frame.ensure(sizeof(double4) * 1024 + 64) || w.isrep.ensure(4 * N) ||
+        w.nodeid.ensure(4 * N) || w.node_child.ensure(sizeof(BHNode) * N) || w.rep.ensure(16 * N) ||
+        w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_cxy.ensure(16 * N))
+        return BL_E_NOMEM;
+    w.n_alloc = n;
+    return BL_OK;
+}
+
+// Builds the tree of the positions d_pos (device, n points) into w; returns BL_OK and
+// leaves the status word in misc[0] (checked by the caller when it needs it).
+int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *d_xs, uint32_t *d_ys) {
+    int rc = ensure_bh(w, n);
+    if (rc) return rc;
+    int *misc = w.misc.as<int>();
+    double *frame = w.frame.as<double>();
+    double4 *fpart = reinterpret_cast<double4 *>(frame + 8);
+    BL_HIP(hipMemsetAsync(misc, 0, 16, s));
+    const int gf = std::min(1024, (n + WG - 1) / WG);
+    hipLaunchKernelGGL(k_frame_partial, dim3(gf), dim3(WG), 0, s, d_pos, n, fpart);
+    hipLaunchKernelGGL(k_frame_final, dim3(1), dim3(WG), 0, s, fpart, gf, frame);
+    uint64_t *ka = w.keys_a.as<uint64_t>(), *kb = w.keys_b.as<uint64_t>();
+    int *ia = w.idx_a.as<int>(), *ib = w.idx_b.as<int>();
+    const int g1 = (n + WG - 1) / WG;
+    hipLaunchKernelGGL(k_keys, dim3(g1), dim3(WG), 0, s, d_pos, n, frame, ka, ia, d_xs, d_ys);
+    const int nblk = (n + SORT_TILE - 1) / SORT_TILE;
+    int *hist = w.hist.as<int>();
+    int *offs = w.scan_tmp.as<int>();
+    int *bsum = misc + 64;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(WG), 0, s, ka, n, shift, nblk, hist);
+        rc = scan_excl(hist, 256 * nblk, offs, bsum, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(WG), 0, s, ka, ia, kb, ib, n, shift, nblk, offs);
+        std::swap(ka, kb);
+        std::swap(ia, ib);
+    }
+    // after 8 passes the sorted data is back in keys_a / idx_a
+    double2 *spos = w.spos.as<double2>();
+    hipLaunchKernelGGL(k_gather, dim3(g1), dim3(WG), 0, s, d_pos, ia, n, spos);
+    if (n >= 2) {
+        int *isrep = w.isrep.as<int>(), *nodeid = w.nodeid.as<int>();
+        BHNode *nodes = w.node_child.as<BHNode>();
+        hipLaunchKernelGGL(k_tree_bounds, dim3(g1), dim3(WG), 0, s, ka, n, isrep, misc);
+        rc = scan_excl(isrep, n - 1, nodeid, bsum, s);
+        if (rc) return rc;
+        const int nsb = (n - 1 + SCAN_TILE - 1) / SCAN_TILE;
+        hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, s, ka, n, nodeid, bsum + nsb, misc);
+        hipLaunchKernelGGL(k_tree_nodes, dim3(g1), dim3(WG), 0, s, ka, n, isrep, nodeid, nodes, misc, misc);
+        // n-1 boundaries give at most n-1 nodes; threads beyond the node count (misc[2]) exit
+        hipLaunchKernelGGL(k_centroid, dim3(g1), dim3(WG), 0, s, nodes, spos, misc);
+    } else {
+        const int neg = -1;
+        BL_HIP(hipMemcpyAsync(misc + 1, &neg, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    BL_HIP(hipGetLastError());
+    return BL_OK;
+}
+
+int launch_rep(BHWork &w, int n, double theta, int precision, hipStream_t s) {
+    const int waves = (n + 63) / 64;
+    const int g = (waves + WG / 64 - 1) / (WG / 64);
+    if (precision == BL_PREC_FP64)
+        hipLaunchKernelGGL(k_rep64, dim3(g), dim3(WG), 0, s, w.spos.as<double2>(), w.idx_a.as<int>(), n,
+                           w.spos.as<double2>(), w.node_child.as<BHNode>(), w.misc.as<int>(), w.frame.as<double>(), theta,
+                           w.rep.as<double2>(), w.misc.as<int>());
+    else
+        hipLaunchKernelGGL(k_rep32, dim3(g), dim3(WG), 0, s, w.spos.as<double2>(), w.idx_a.as<int>(), n,
+                           w.spos.as<double2>(), w.node_child.as<BHNode>(), w.misc.as<int>(), w.frame.as<double>(), theta,
+                           w.rep.as<double2>(), w.misc.as<int>());
+    BL_HIP(hipGetLastError());
+    return BL_OK;
+}
+
+}  // namespace
+
+int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta, bl_stats *st) {
+    const int n = g->n;
+    hipStream_t s = g->stream;
+    double2 *pos = reinterpret_cast<double2 *>(d_xy);
+    if (g->eterm.ensure(sizeof(double) * (size_t)n) || g->energy.ensure(sizeof(double) * 4) ||
+        g->scratch.ensure(sizeof(double2) * (size_t)n))
+        return BL_E_NOMEM;
+    int rc = ensure_bh(g->bh, n);
+    if (rc) return rc;
+    double STEP = step0;
+    for (int it = 0; it < iters; ++it) {
+        rc = build_tree(g->bh, pos, n, s, nullptr, nullptr);  // tree of the iteration-start positions
+        if (rc) return rc;
+        rc = launch_rep(g->bh, n, theta, p->precision, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_attract_move, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
+                           g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
+                           g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
+        rc = launch_energy(g->eterm.as<double>(), n, g->energy.as<double>(), p->precision, s);
+        if (rc) return rc;
+        STEP = STEP * 0.999;  // :1070
+    }
+    int status = 0;
+    double E = 1.79769313486231570815e+308;  // ENERGY starts at DBL_MAX (:1024)
+    if (iters > 0) {
+        BL_HIP(hipMemcpyAsync(&status, g->bh.misc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
+        BL_HIP(hipMemcpyAsync(&E, g->energy.as<double>(), sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    BL_HIP(hipStreamSynchronize(s));
+    if (status & ST_DUPKEY) return BL_E_DUPKEY;
+    if (st) {
+        st->energy = E;
+        st->iters_done = iters;
+        st->step_final = STEP;
+    }
+    return BL_OK;
+}
+
+// ---- host entry points for the pieces ----------------------------------------------------
+namespace {
+struct Scoped {
+    hipStream_t s = nullptr;
+    BHWork w;
+    void *buf = nullptr;
+    ~Scoped() {
+        if (s) (void)hipStreamSynchronize(s);
+        w.release();
+        if (buf) (void)hipFree(buf);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+}  // namespace
+
+int bh_morton_host(int32_t n, const double *xy, uint32_t *xs, uint32_t *ys, double *maxmin) {
+    Scoped sc;
+    BL_HIP(hipStreamCreateWithFlags(&sc.s, hipStreamNonBlocking));
+    const size_t pb = 16 * (size_t)n;
+    BL_HIP(hipMalloc(&sc.buf, pb + 8 * (size_t)n));
+    double2 *dpos = static_cast<double2 *>(sc.buf);
+    uint32_t *dxs = reinterpret_cast<uint32_t *>(static_cast<char *>(sc.buf) + pb);
+    uint32_t *dys = dxs + n;
+    BL_HIP(hipMemcpyAsync(dpos, xy, pb, hipMemcpyHostToDevice, sc.s));
+    int rc = build_tree(sc.w, dpos, n, sc.s, dxs, dys);
+    if (rc) return rc;
+    BL_HIP(hipMemcpyAsync(xs, dxs, 4 * (size_t)n, hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipMemcpyAsync(ys, dys, 4 * (size_t)n, hipMemcpyDeviceToHost, sc.s));
+    double fr[3];
+    BL_HIP(hipMemcpyAsync(fr, sc.w.frame.p, sizeof(fr), hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipStreamSynchronize(sc.s));
+    if (maxmin) *maxmin = fr[2];
+    return BL_OK;
+}
+
+int bh_tree_host(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, int32_t *nnodes, int32_t *root,
+                 double *diam, double *cx, double *cy, int32_t *child) {
+    Scoped sc;
+    BL_HIP(hipStreamCreateWithFlags(&sc.s, hipStreamNonBlocking));
+    const size_t pb = 16 * (size_t)n;
+    BL_HIP(hipMalloc(&sc.buf, pb));
+    double2 *dpos = static_cast<double2 *>(sc.buf);
+    BL_HIP(hipMemcpyAsync(dpos, xy, pb, hipMemcpyHostToDevice, sc.s));
+    int rc = build_tree(sc.w, dpos, n, sc.s, nullptr, nullptr);
+    if (rc) return rc;
+    int misc[3];
+    double fr[3];
+    BL_HIP(hipMemcpyAsync(misc, sc.w.misc.p, sizeof(misc), hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipMemcpyAsync(fr, sc.w.frame.p, sizeof(fr), hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipMemcpyAsync(perm, sc.w.idx_a.p, 4 * (size_t)n, hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipStreamSynchronize(sc.s));
+    const int m = n >= 2 ? misc[2] : 0;
+    if (misc[0] & ST_DUPKEY) return BL_E_DUPKEY;
+    *nnodes = m;
+    *root = n >= 2 ? misc[1] : -1;
+    if (m > max_nodes) return BL_E_ARG;
+    if (m > 0) {
+        std::vector<BHNode> h(m);
+        BL_HIP(hipMemcpyAsync(h.data(), sc.w.node_child.p, sizeof(BHNode) * m, hipMemcpyDeviceToHost, sc.s));
+        BL_HIP(hipStreamSynchronize(sc.s));
+        for (int v = 0; v < m; ++v) {
+            if (diam) diam[v] = ldexp(fr[2], -h[v].depth);
+            if (cx) cx[v] = h[v].c.x;
+            if (cy) cy[v] = h[v].c.y;
+            const int c[4] = {h[v].ch.x, h[v].ch.y, h[v].ch.z, h[v].ch.w};
+            for (int q = 0; q < 4 && child; ++q)
+                child[4 * v + q] = c[q] <= -2 ? -(perm[-c[q] - 2] + 2) : c[q];
+        }
+    }
+    return BL_OK;
+}
+
+int bh_forces_host(int32_t n, const double *xy, double theta, int32_t nq, const double *q, double *out,
+                   int32_t precision) {
+    // Builds the tree of xy, then traverses it for the nq query positions.
+    Scoped sc;
+    BL_HIP(hipStreamCreateWithFlags(&sc.s, hipStreamNonBlocking));
+    const size_t pb = 16 * (size_t)n, qb = 16 * (size_t)nq;
+    BL_HIP(hipMalloc(&sc.buf, pb + qb + 4 * (size_t)nq + 16 * (size_t)nq));
+    double2 *dpos = static_cast<double2 *>(sc.buf);
+    double2 *dq = reinterpret_cast<double2 *>(static_cast<char *>(sc.buf) + pb);
+    int *qid = reinterpret_cast<int *>(static_cast<char *>(sc.buf) + pb + qb);
+    double2 *dout = reinterpret_cast<double2 *>(static_cast<char *>(sc.buf) + pb + qb + 4 * (size_t)nq);
+    BL_HIP(hipMemcpyAsync(dpos, xy, pb, hipMemcpyHostToDevice, sc.s));
+    if (nq > 0) BL_HIP(hipMemcpyAsync(dq, q, qb, hipMemcpyHostToDevice, sc.s));
+    int rc = build_tree(sc.w, dpos, n, sc.s, nullptr, nullptr);
+    if (rc) return rc;
+    if (nq == 0) return BL_OK;
+    std::vector<int> ids(nq);
+    for (int k = 0; k < nq; ++k) ids[k] = k;
+    BL_HIP(hipMemcpyAsync(qid, ids.data(), 4 * (size_t)nq, hipMemcpyHostToDevice, sc.s));
+    const int waves = (nq + 63) / 64;
+    const int gq = (waves + WG / 64 - 1) / (WG / 64);
+    BHWork &w = sc.w;
+    if (precision == BL_PREC_FP64)
+        hipLaunchKernelGGL(k_rep64, dim3(gq), dim3(WG), 0, sc.s, dq, qid, nq, w.spos.as<double2>(), w.node_child.as<BHNode>(),
+                           w.misc.as<int>(), w.frame.as<double>(), theta, dout, w.misc.as<int>());
+    else
+        hipLaunchKernelGGL(k_rep32, dim3(gq), dim3(WG), 0, sc.s, dq, qid, nq, w.spos.as<double2>(), w.node_child.as<BHNode>(),
+                           w.misc.as<int>(), w.frame.as<double>(), theta, dout, w.misc.as<int>());
+    BL_HIP(hipGetLastError());
+    int status = 0;
+    BL_HIP(hipMemcpyAsync(&status, w.misc.p, sizeof(int), hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipMemcpyAsync(out, dout, 16 * (size_t)nq, hipMemcpyDeviceToHost, sc.s));
+    BL_HIP(hipStreamSynchronize(sc.s));
+    if (status & ST_DUPKEY) return BL_E_DUPKEY;
+    return BL_OK;
+}
+
+}  // namespace bl

@@ -1,0 +1,86 @@
+// Internal declarations shared by the HIP translation units of libbatchlayout.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "batchlayout.h"
+
+#define BL_HIP(call)                                                      \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return bl::device_error(e_, #call, __LINE__); \
+    } while (0)
+
+namespace bl {
+
+int device_error(hipError_t e, const char *what, int line);
+
+// Attraction laws of the minibatch family (the only difference between algos 2, 3, 8).
+enum Law : int { LAW_FR = 2, LAW_LINLOG = 3, LAW_FA = 8 };
+
+// Device buffer that grows on demand and is freed with its owner.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Barnes-Hut workspace (sized to n on first use).
+struct BHWork {
+    DevBuf keys_a, keys_b, idx_a, idx_b;     // radix sort ping-pong (uint64 keys, int32 ids)
+    DevBuf hist;                             // per-block digit histograms
+    DevBuf spos;                             // positions in Morton order (double2)
+    DevBuf frame;                            // minx, miny, maxmin (+ block partials)
+    DevBuf isrep, nodeid;                    // boundary representative flags and node ids
+    DevBuf bnd_lo;                           // node range start per boundary
+    DevBuf node_child, node_depth, node_cnt, node_parent, node_cxy, node_rep;
+    DevBuf scan_tmp;
+    DevBuf rep;                              // repulsion result per vertex (double2)
+    DevBuf misc;                             // counters / status words
+    int32_t n_alloc = 0;
+    void release();
+};
+
+}  // namespace bl
+
+struct bl_graph {
+    int32_t n = 0;
+    int64_t nnz = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // CSR as given (algo 4 attraction walks every entry, algorithms.cpp:1055-1058)
+    bl::DevBuf rowptr, colids, w_edge;
+    bool has_w = false;
+    // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),
+    // as a CSR of strictly increasing column ids per row (+ their weights)
+    bl::DevBuf eff_rowptr, eff_colids, eff_w;
+    int32_t eff_max_deg = 0;
+    // workspace
+    bl::DevBuf pos, part, eterm, energy, scratch;
+    bl::BHWork bh;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace bl {
+
+// minibatch.hip
+int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0,
+                  int iters, bool converged, bl_stats *st);
+// bh.hip
+int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta,
+           bl_stats *st);
+int bh_morton_host(int32_t n, const double *xy, uint32_t *xs, uint32_t *ys, double *maxmin);
+int bh_tree_host(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, int32_t *nnodes,
+                 int32_t *root, double *diam, double *cx, double *cy, int32_t *child);
+int bh_forces_host(int32_t n, const double *xy, double theta, int32_t nq, const double *q, double *out,
+                   int32_t precision);
+// batched.hip
+int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,
+                const int32_t *colids, double *xy, const bl_params *p, double *energies, bl_stats *st);
+
+}  // namespace bl

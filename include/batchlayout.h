@@ -1,0 +1,155 @@
+/* batchlayout.h -- C ABI of the MI355X-native BatchLayout library (libbatchlayout.so).
+ *
+ * The reference (khaled-rahman/BatchLayoutCode) is a CLI program with no library
+ * or FFI.  Its in-process seam is the `algorithms` class (sample/algorithms.h:50-85):
+ * a constructor that copies the CSR graph (sample/algorithms.cpp:3-15) and one
+ * method per layout algorithm that mutates the coordinates in place and returns
+ * {ENERGY, seconds}.  This header is that seam as plain C: a graph handle
+ * replaces the constructor, bl_run() replaces the layout methods, and the .mtx /
+ * -initf / output-file helpers replace IO.h and algorithms.cpp's file code.
+ * Every entry point names the reference interface it replaces.
+ *
+ * Conventions: positions are n x 2 row-major FP64 (x0,y0,x1,y1,...), the same
+ * bytes as the reference's Coordinate<double> array (sample/Coordinate.h:26-85).
+ * CSR arrays are 0-based int32 with ascending column ids per row, as built by
+ * IO.h:60-155 -> CSC.h:147-188 -> CSR.h:155-186.  All functions return BL_OK (0)
+ * or a negative BL_E* status; none exits the process or asserts (the reference
+ * calls exit(1)/assert instead).  Calls on different graph handles may run
+ * concurrently from different host threads; one handle is used by one thread
+ * at a time.
+ */
+#ifndef BATCHLAYOUT_H
+#define BATCHLAYOUT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------- */
+#define BL_OK 0
+#define BL_E_ARG (-1)        /* invalid argument (null pointer, n <= 0, batch <= 0, bad algo ...) */
+#define BL_E_IO (-2)         /* file cannot be opened / read / written */
+#define BL_E_FORMAT (-3)     /* malformed .mtx (count mismatch, index out of range, non-square) */
+#define BL_E_DEVICE (-4)     /* HIP runtime error or no usable gfx950 device */
+#define BL_E_DUPKEY (-5)     /* Barnes-Hut: two vertices share a Morton cell (the reference recurses forever) */
+#define BL_E_NOMEM (-6)      /* host or device allocation failed */
+#define BL_E_GRAPH (-7)      /* CSR fails validation (rowptr not monotone, colid out of range) */
+
+/* ---- algorithms (the reference's -algo values, Test/BatchLayout.cpp:132-174) ---- */
+#define BL_ALGO_CACHE 2      /* cacheBlockingminiBatchForceDirectedAlgorithm, algorithms.cpp:715-794 */
+#define BL_ALGO_LINLOG 3     /* LinLogcacheBlockingminiBatchForceDirectedAlgorithm, :874-945 */
+#define BL_ALGO_BH 4         /* BarnesHutApproximation, :1020-1083 */
+#define BL_ALGO_BHCACHE 6    /* approxCacheBlockBH (80 % BH, then 20 % algo 2), :1192-1215 */
+#define BL_ALGO_FA 8         /* FAcacheBlockingminiBatchForceDirectedAlgorithm, :947-1018 */
+
+/* ---- precision of the pair arithmetic -------------------------------------------- */
+#define BL_PREC_FP32 0       /* fast path: FP32 pair math, FP64 coordinate differences, FP64 tile sums */
+#define BL_PREC_FP64 1       /* reference-grade path: the reference's FP64 operations in its order */
+
+typedef struct bl_graph bl_graph;
+
+/* Layout parameters.  Defaults (bl_params_default) are the reference CLI defaults,
+ * Test/BatchLayout.cpp:57-58: -weight 1.0 -bht 1.2 -engt 0.01 -batch 256 -iter 600. */
+typedef struct {
+    int32_t algo;       /* BL_ALGO_* */
+    int32_t iters;      /* -iter */
+    int32_t batch;      /* -batch (minibatch size B) */
+    int32_t converged;  /* algo 2 only: run until |E0-E|/E0 <= engt, ignoring iters
+                           (cacheBlockingminiBatchForceDirectedAlgorithmConverged, :795-871;
+                           the CLI selects it when -engt != 0.01, BatchLayout.cpp:147-151) */
+    double bht;         /* -bht, Barnes-Hut opening threshold theta */
+    double engt;        /* -engt */
+    double weight;      /* -weight, the global attraction scale W */
+    double step0;       /* STEP of the first iteration (1.0 as in the reference; a resumed
+                           or teacher-forced run passes the STEP it continues from) */
+    int32_t precision;  /* BL_PREC_* */
+    int32_t reserved;
+} bl_params;
+
+typedef struct {
+    double energy;        /* sum_i |F_i|^2 of the last iteration (the reference's ENERGY) */
+    int32_t iters_done;   /* iterations run (differs from iters in converged mode) */
+    int32_t status;       /* BL_OK, or the first error met */
+    double seconds;       /* wall time of the layout loop (the reference's end - start) */
+    double step_final;    /* STEP after the last iteration */
+} bl_stats;
+
+/* Fills *p with the reference CLI defaults. */
+int bl_params_default(bl_params *p);
+
+/* Replaces the algorithms constructor (algorithms.cpp:3-15), which deep-copies the CSR.
+ * Copies rowptr[n+1], colids[nnz] and optional per-entry weights w_edge[nnz] (nullable;
+ * an extension: the reference multiplies every attraction by the global W only) to the
+ * device.  Validates the CSR (BL_E_GRAPH).  Rows need not be sorted or duplicate-free:
+ * the reference's neighbour test (algorithms.cpp:758-762) is reproduced for any input. */
+int bl_graph_create(int32_t n, int64_t nnz, const int32_t *rowptr, const int32_t *colids,
+                    const float *w_edge, bl_graph **out);
+int bl_graph_destroy(bl_graph *g);
+int bl_graph_size(const bl_graph *g, int32_t *n, int64_t *nnz);
+
+/* Replaces the layout methods (algorithms.h:67-80): runs the layout on host positions
+ * xy_inout[2n] (updated in place), synchronously.  Keeps no pointer to caller memory. */
+int bl_run(bl_graph *g, double *xy_inout, const bl_params *p, bl_stats *st);
+
+/* Same, on positions already resident in device memory (d_xy: 2n doubles allocated by
+ * the caller with hipMalloc); `stream` is a hipStream_t or NULL for the library's own.
+ * Returns after the layout has finished. */
+int bl_run_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats *st, void *stream);
+
+/* ---- host I/O: the reference's file formats ------------------------------------- */
+/* .mtx -> CSR, replacing SetInputMatricesAsCSR (sample/commonutility.h:44-53) =
+ * ReadASCII (IO.h:60-155) + CSC(Triple*) (CSC.h:147-188) + CSR(const CSC&) (CSR.h:155-186).
+ * *rowptr and *colids are allocated with malloc; release them with bl_free. */
+int bl_mtx_read(const char *path, int32_t *n, int64_t *nnz, int32_t **rowptr, int32_t **colids);
+void bl_free(void *p);
+
+/* -initf reader, replacing algorithms::fileInitialization (algorithms.cpp:98-119): one
+ * "x y id" line per vertex, in vertex order (the id column is ignored).  Reads at most n
+ * lines (the reference overflows its array on more); *lines_read reports how many. */
+int bl_initf_read(const char *path, int32_t n, double *xy, int32_t *lines_read);
+
+/* Output writer, replacing algorithms::writeToFile (algorithms.cpp:1244-1256):
+ * "x<TAB>y<TAB>id" with 1-based ids and 6 significant digits (default ostream). */
+int bl_output_write(const char *path, int32_t n, const double *xy);
+
+/* ---- Barnes-Hut pieces (for tooling and parity checks) -------------------------- */
+/* Morton scales of MortonCode.h:42-67 on the device: xs[i] = xscale(x_i), ys[i] =
+ * yscale(y_i), *maxmin = MAXMIN.  Host buffers. */
+int bl_bh_morton(int32_t n, const double *xy, uint32_t *xs, uint32_t *ys, double *maxmin);
+
+/* Sort permutation and compressed quadtree of BarnesHut's constructor and build_tree
+ * (BarnesHut.h:114-220), built on the device.  perm[k] = vertex at Morton rank k.
+ * The tree is the reference's quadtree with single-child chains collapsed into their
+ * lowest node (chain nodes share one centroid, so the traversal is unchanged).
+ * child[4v+q]: >= 0 node, -1 empty quadrant, <= -2 leaf holding vertex (-c-2).
+ * diam[v] is the node's cell diameter; *root is the root node id (or -1 when n == 1).
+ * Arrays hold max_nodes entries (n-1 always suffices). */
+int bl_bh_tree(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, int32_t *nnodes,
+               int32_t *root, double *diam, double *cx, double *cy, int32_t *child);
+
+/* calcRepForce (BarnesHut.h:69-86, :221-225) on the device: for each query q_k, the sum of
+ * unit vectors toward accepted cells and leaves of the tree of the n points xy. */
+int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t nq, const double *q,
+                 double *out, int32_t precision);
+
+/* ---- batched small graphs -------------------------------------------------------- */
+/* Lays out `count` independent graphs in one persistent launch (no counterpart in the
+ * reference, which lays out one graph per process).  Graph k has vertices
+ * [voff[k], voff[k+1]) and CSR entries [eoff[k], eoff[k+1]) of the concatenated arrays;
+ * rowptr is global (rowptr[voff[k]] == eoff[k]) and colids are local to the graph.
+ * Every graph runs algo p->algo (2 or 8) with p->batch/iters/weight; energies[k] receives
+ * each graph's final energy.  Graphs up to BL_BATCHED_MAX_N vertices. */
+#define BL_BATCHED_MAX_N 8192
+int bl_run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,
+                   const int32_t *colids, double *xy_inout, const bl_params *p, double *energies,
+                   bl_stats *st);
+
+/* Library/device information: writes a short NUL-terminated description. */
+int bl_device_info(char *buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BATCHLAYOUT_H */

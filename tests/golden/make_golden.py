@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the reference's own code.
+
+Run only where the reference checkout exists (it builds oracle/_ref/ from the
+reference sources with oracle/Makefile, then drives oracle/ref_harness.cpp):
+
+    python tests/golden/make_golden.py [--ref /root/reference]
+
+Outputs (all small, committed):
+  inputs/*.mtx          parity inputs: four small .mtx files from the reference's
+                        datasets/input/, synthetic grids, and .mtx edge cases
+  csr_<name>.npz        rowptr/colids the reference builds from each input (IO.h, CSC.h, CSR.h)
+  traj_<case>.npz       reference coordinates after k iterations (one run per k, %.17g),
+                        fast-math build ("xy") and strict-IEEE build ("xy_strict")
+  band_<case>.npz       final energies of runs from 1-ulp-perturbed inits (FP64 noise band)
+  bh_<case>.npz         Morton scales, sorted permutation, quadtree (pre-order), BH forces
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from batchlayoutcode_amd import graphs  # noqa: E402
+
+INP = os.path.join(HERE, "inputs")
+REF_INPUTS = ["test.mtx", "test2.mtx", "bcspwr01.mtx", "jagmesh1.mtx"]
+
+
+def sh(cmd, **kw):
+    return subprocess.run(cmd, check=True, capture_output=True, text=True, **kw).stdout
+
+
+def ref_run(binary, mtx, initf, algo, iters, batch, bht=1.2, weight=1.0, engt=0.01, threads=8):
+    out = sh([binary, "run", mtx, initf, str(algo), str(iters), str(batch), repr(bht),
+              repr(weight), repr(engt), str(threads)], cwd="/tmp")
+    lines = out.splitlines()
+    k = next(i for i, ln in enumerate(lines) if ln.startswith("E "))
+    e = float(lines[k].split()[1])
+    xy = np.array([[float(v) for v in ln.split()] for ln in lines[k + 1:] if ln.strip()])
+    return e, xy
+
+
+def ref_csr(binary, mtx):
+    out = sh([binary, "csr", mtx], cwd="/tmp").splitlines()
+    k = next(i for i, ln in enumerate(out) if ln.startswith("CSR "))
+    n, nnz = map(int, out[k].split()[1:])
+    vals = np.array([int(v) for v in out[k + 1:k + 1 + n + 1 + nnz]], dtype=np.int64)
+    return vals[:n + 1].astype(np.int32), vals[n + 1:].astype(np.int32)
+
+
+def ref_bh(binary, pts, theta):
+    path = "/tmp/_bl_golden_pts.txt"
+    with open(path, "w") as f:
+        for x, y in pts:
+            f.write("%.17g %.17g\n" % (x, y))
+    out = sh([binary, "bh", path, repr(theta)]).splitlines()
+    rec = {"S": [], "P": [], "N": [], "L": [], "F": []}
+    maxmin = None
+    for ln in out:
+        t = ln.split()
+        if not t:
+            continue
+        if t[0] == "M":
+            maxmin = float(t[1])
+        elif t[0] in rec:
+            rec[t[0]].append(t[1:])
+    nodes = np.array(rec["N"], dtype=object)
+    return dict(
+        maxmin=np.float64(maxmin),
+        scales=np.array(rec["S"], dtype=np.uint64).astype(np.uint32),
+        perm=np.array(rec["P"], dtype=np.int64).ravel().astype(np.int32),
+        node_id=nodes[:, 0].astype(np.int32), node_parent=nodes[:, 1].astype(np.int32),
+        node_quad=nodes[:, 2].astype(np.int32), node_diam=nodes[:, 3].astype(np.float64),
+        node_cx=nodes[:, 4].astype(np.float64), node_cy=nodes[:, 5].astype(np.float64),
+        leaf=np.array(rec["L"], dtype=np.int64).astype(np.int32),
+        force=np.array(rec["F"], dtype=np.float64),
+    )
+
+
+def write_edge_cases():
+    cases = {
+        # general (unsymmetric) header: entries taken as-is, self-loops kept, real values
+        "general_selfloop.mtx": "%%MatrixMarket matrix coordinate real general\n"
+                                "4 4 7\n1 2 3.5\n2 1 1\n2 2 4\n3 4 2\n4 3 2\n1 3 1\n3 1 1\n",
+        # symmetric with diagonal entries (dropped) and integer values
+        "sym_diag.mtx": "%%MatrixMarket matrix coordinate integer symmetric\n"
+                        "5 5 7\n1 1 9\n2 1 1\n3 2 1\n3 3 2\n4 3 1\n5 4 1\n5 1 1\n",
+        # "symmetric" only in a later comment line still turns on mirroring (IO.h:67-75)
+        "sym_late_comment.mtx": "%%MatrixMarket matrix coordinate pattern general\n"
+                                "% the structure is symmetric\n4 4 3\n2 1\n3 2\n4 3\n",
+        # duplicate entry in a symmetric file: kept twice (no merging)
+        "sym_duplicate.mtx": "%%MatrixMarket matrix coordinate pattern symmetric\n"
+                             "4 4 5\n2 1\n3 2\n2 1\n4 3\n4 1\n",
+    }
+    for name, text in cases.items():
+        with open(os.path.join(INP, name), "w") as f:
+            f.write(text)
+    return list(cases)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--skip-long", action="store_true")
+    args = ap.parse_args()
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.ref}"], check=True)
+    fast = os.path.join(ROOT, "oracle", "_ref", "ref_fast")
+    strict = os.path.join(ROOT, "oracle", "_ref", "ref_strict")
+    os.makedirs(INP, exist_ok=True)
+    for name in REF_INPUTS:
+        shutil.copy(os.path.join(args.ref, "datasets", "input", name), os.path.join(INP, name))
+    n8, e8 = graphs.grid(8, 8)
+    graphs.write_mtx(os.path.join(INP, "grid8x8.mtx"), n8, e8)
+    n64, e64 = graphs.grid(64, 64)
+    graphs.write_mtx(os.path.join(INP, "grid64x64.mtx"), n64, e64)
+    edge_cases = write_edge_cases()
+
+    # --- CSR built by the reference ------------------------------------------------
+    for name in REF_INPUTS + ["grid8x8.mtx", "grid64x64.mtx"] + edge_cases:
+        rp, ci = ref_csr(fast, os.path.join(INP, name))
+        np.savez_compressed(os.path.join(HERE, f"csr_{name[:-4]}.npz"), rowptr=rp, colids=ci)
+
+    # --- trajectories --------------------------------------------------------------
+    def n_of(name):
+        rp, _ = ref_csr(fast, os.path.join(INP, name))
+        return len(rp) - 1
+
+    traj_cases = [
+        # (case, input, algo, batch, ks, extra kwargs, init seed)
+        ("a2_test_b256", "test.mtx", 2, 256, [1, 2, 3, 5, 10], {}, 0),
+        ("a2_test2_b4", "test2.mtx", 2, 4, [1, 2, 3, 10], {}, 0),
+        ("a2_bcspwr01_b8_w2", "bcspwr01.mtx", 2, 8, [1, 2, 3, 10], {"weight": 2.0}, 5),
+        ("a2_grid8_b16", "grid8x8.mtx", 2, 16, [1, 2, 3, 10, 50], {}, 0),
+        ("a2_jagmesh1_b256", "jagmesh1.mtx", 2, 256, [1, 2, 3], {}, 0),
+        ("a2_grid64_b256", "grid64x64.mtx", 2, 256, list(range(1, 11)), {}, 0),
+        ("a8_test2_b4", "test2.mtx", 8, 4, [1, 2, 3, 10], {}, 0),
+        ("a8_grid64_b256", "grid64x64.mtx", 8, 256, [1, 2, 3, 5], {}, 0),
+        ("a3_grid8_b16", "grid8x8.mtx", 3, 16, [1, 2, 3, 10], {}, 0),
+        ("a21_test2_b4_e05", "test2.mtx", 21, 4, [0], {"engt": 0.05}, 0),
+        ("a21_grid8_b16_e02", "grid8x8.mtx", 21, 16, [0], {"engt": 0.02}, 0),
+        ("a4_test_b256", "test.mtx", 4, 256, [1, 2, 3, 10], {}, 0),
+        ("a4_grid8_b16_t05", "grid8x8.mtx", 4, 16, [1, 2, 3, 10], {"bht": 0.5}, 0),
+        ("a4_jagmesh1_b64", "jagmesh1.mtx", 4, 64, [1, 2, 3], {}, 0),
+        ("a4_grid64_b256", "grid64x64.mtx", 4, 256, list(range(1, 11)), {}, 0),
+        ("a6_grid8_b16", "grid8x8.mtx", 6, 16, [10], {}, 0),
+    ]
+    for case, name, algo, batch, ks, kw, seed in traj_cases:
+        mtx = os.path.join(INP, name)
+        n = n_of(name)
+        init = graphs.uniform_init(n, seed)
+        initf = f"/tmp/_bl_golden_init_{case}.txt"
+        graphs.write_initf(initf, init)
+        thr = 1 if algo in (4, 6) else 8
+        xs, es, xs2, es2 = [], [], [], []
+        for k in ks:
+            e, xy = ref_run(fast, mtx, initf, algo, k, batch, threads=thr, **kw)
+            xs.append(xy)
+            es.append(e)
+            e2, xy2 = ref_run(strict, mtx, initf, algo, k, batch, threads=thr, **kw)
+            xs2.append(xy2)
+            es2.append(e2)
+        np.savez_compressed(os.path.join(HERE, f"traj_{case}.npz"), input=name, algo=algo, batch=batch,
+                            ks=np.array(ks), init=init, xy=np.array(xs), E=np.array(es),
+                            xy_strict=np.array(xs2), E_strict=np.array(es2),
+                            weight=kw.get("weight", 1.0), bht=kw.get("bht", 1.2), engt=kw.get("engt", 0.01))
+        print("traj", case, "E", es)
+
+    # --- energy noise band: 600 iterations from the grid64 init and 1-ulp perturbations
+    if not args.skip_long:
+        for case, algo, thr in (("a2_grid64_b256_600", 2, 8), ("a4_grid64_b256_600", 4, 1),
+                                ("a8_grid64_b256_600", 8, 8)):
+            mtx = os.path.join(INP, "grid64x64.mtx")
+            init = graphs.uniform_init(4096, 0)
+            es, es_strict = [], []
+            final = None
+            for p in range(6):
+                xy0 = init.copy()
+                if p > 0:
+                    rng = np.random.default_rng(100 + p)
+                    xy0 = np.nextafter(xy0, xy0 + rng.choice([-1.0, 1.0], size=xy0.shape))
+                initf = f"/tmp/_bl_golden_band_{p}.txt"
+                graphs.write_initf(initf, xy0)
+                e, xy = ref_run(fast, mtx, initf, algo, 600, 256, threads=thr)
+                es.append(e)
+                if p == 0:
+                    final = xy
+                    e2, _ = ref_run(strict, mtx, initf, algo, 600, 256, threads=thr)
+                    es_strict.append(e2)
+            np.savez_compressed(os.path.join(HERE, f"band_{case}.npz"), E=np.array(es),
+                                E_strict=np.array(es_strict), xy_final=final)
+            print("band", case, es, es_strict)
+
+    # --- Barnes-Hut structures on point clouds ---------------------------------------
+    rng = np.random.default_rng(11)
+    blobs = np.concatenate([rng.normal(c, s, size=(1000, 2))
+                            for c, s in (((0, 0), 0.05), ((1, 1), 0.2), ((3, -1), 0.01),
+                                         ((-2, 2), 1.0), ((0.5, 0.5), 0.001))])
+    lattice = np.stack(np.meshgrid(np.arange(40.0), np.arange(25.0)), -1).reshape(-1, 2)
+    bh_cases = [
+        ("u1000_t12", np.random.default_rng(21).random((1000, 2)), 1.2),
+        ("u20000_t12", np.random.default_rng(22).random((20000, 2)), 1.2),
+        ("u2000_t03", np.random.default_rng(23).random((2000, 2)) * 100.0 - 50.0, 0.3),
+        ("u2000_t20", np.random.default_rng(24).random((2000, 2)), 2.0),
+        ("blobs5000_t05", blobs, 0.5),
+        ("lattice1000_t12", lattice, 1.2),
+        ("grid64init_t12", graphs.uniform_init(4096, 0), 1.2),
+        ("tiny3_t12", np.array([[0.0, 0.0], [1.0, 0.5], [0.25, 1.0]]), 1.2),
+    ]
+    for case, pts, theta in bh_cases:
+        d = ref_bh(fast, pts, theta)
+        d2 = ref_bh(strict, pts, theta)
+        np.savez_compressed(os.path.join(HERE, f"bh_{case}.npz"), pts=pts, theta=theta,
+                            force_strict=d2["force"], scales_strict=d2["scales"], perm_strict=d2["perm"], **d)
+        same = (np.array_equal(d["perm"], d2["perm"]) and np.array_equal(d["node_cx"], d2["node_cx"]))
+        print("bh", case, "nodes", len(d["node_id"]), "fast==strict tree", same,
+              "max|dF|", np.abs(d["force"] - d2["force"]).max())
+
+
+if __name__ == "__main__":
+    main()
