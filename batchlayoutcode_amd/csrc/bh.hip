@@ -767,19 +767,32 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         return BL_E_NOMEM;
     int rc = ensure_bh(g->bh, n);
     if (rc) return rc;
+    Prof prof;
+    prof.on = p->profile != 0;
+    prof.s = s;
     double STEP = step0;
     for (int it = 0; it < iters; ++it) {
+        prof.begin(3);
         rc = build_tree(g->bh, pos, n, s, nullptr, nullptr);  // tree of the iteration-start positions
+        prof.end();
         if (rc) return rc;
+        prof.begin(0);
         rc = launch_rep(g->bh, n, theta, p->precision, s);
+        prof.end();
         if (rc) return rc;
+        prof.begin(1);
         hipLaunchKernelGGL(k_attract_move, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                            g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                            g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
+        prof.end();
+        prof.begin(2);
         rc = launch_energy(g->eterm.as<double>(), n, g->energy.as<double>(), p->precision, s);
+        prof.end();
         if (rc) return rc;
+        prof.flush();
         STEP = STEP * 0.999;  // :1070
     }
+    prof.report(st);
     int status = 0;
     double E = 1.79769313486231570815e+308;  // ENERGY starts at DBL_MAX (:1024)
     if (iters > 0) {

@@ -68,6 +68,58 @@ struct bl_graph {
 
 namespace bl {
 
+// Optional per-launch timing with HIP events on the launch stream (bl_params.profile).
+struct Prof {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    size_t used = 0;
+    double ms[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
+    void begin(int c) {
+        if (!on) return;
+        if (2 * (used + 1) > ev.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { on = false; return; }
+            ev.push_back(a);
+            ev.push_back(b);
+            cls.push_back(c);
+        }
+        cls[used] = c;
+        (void)hipEventRecord(ev[2 * used], s);
+    }
+    void end() {
+        if (!on) return;
+        (void)hipEventRecord(ev[2 * used + 1], s);
+        ++used;
+    }
+    // waits for the recorded launches and folds their durations into ms/cnt
+    void flush() {
+        if (!on || used == 0) return;
+        (void)hipEventSynchronize(ev[2 * used - 1]);
+        for (size_t k = 0; k < used; ++k) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]) == hipSuccess) {
+                ms[cls[k]] += t;
+                cnt[cls[k]] += 1;
+            }
+        }
+        used = 0;
+    }
+    void report(bl_stats *st) {
+        flush();
+        if (!st) return;
+        for (int c = 0; c < 4; ++c) {
+            st->kernel_ms[c] += ms[c];
+            st->kernel_launches[c] += cnt[c];
+        }
+    }
+    ~Prof() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 // minibatch.hip
 int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0,
                   int iters, bool converged, bl_stats *st);

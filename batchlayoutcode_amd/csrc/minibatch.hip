@@ -64,11 +64,12 @@ __device__ __forceinline__ void attract64(int law, double W, double dx, double d
 
 // Read-only view of the positions in the constant address space: with a wave-uniform
 // index the compiler fetches x_j with scalar loads (one fetch per wave, no VGPRs).
-typedef const __attribute__((address_space(4))) double2 cdouble2;
+typedef const __attribute__((address_space(4))) double cdouble;
 
-__device__ __forceinline__ void pair32(const double2 q, const double px, const double py, float &tx, float &ty) {
-    const float dx = (float)(q.x - px);
-    const float dy = (float)(q.y - py);
+__device__ __forceinline__ void pair32(const double qx, const double qy, const double px, const double py, float &tx,
+                                       float &ty) {
+    const float dx = (float)(qx - px);
+    const float dy = (float)(qy - py);
     const float r2 = fmaf(dy, dy, fmaf(dx, dx, kTiny));
     const float inv = __builtin_amdgcn_rcpf(r2);
     tx = fmaf(-dx, inv, tx);
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos,
     constexpr int IPW = WG / JS;
     const int li = tid % IPW;
     const int js = __builtin_amdgcn_readfirstlane(tid / IPW);  // IPW is a multiple of 64
-    cdouble2 *cpos = (cdouble2 *)pos;
+    cdouble *cpos = (cdouble *)pos;
     const int slot = (int)blockIdx.y * IPW + li;
     const int i = b0 + (slot < bc ? slot : bc - 1);
     const double px = pos[i].x, py = pos[i].y;
@@ -131,9 +132,9 @@ __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos,
         int u = t0;
         for (; u + 8 <= t1; u += 8) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) pair32(cpos[u + k], px, py, tx, ty);
+            for (int k = 0; k < 8; ++k) pair32(cpos[2 * (u + k)], cpos[2 * (u + k) + 1], px, py, tx, ty);
         }
-        for (; u < t1; ++u) pair32(cpos[u], px, py, tx, ty);
+        for (; u < t1; ++u) pair32(cpos[2 * u], cpos[2 * u + 1], px, py, tx, ty);
         ax += (double)tx;
         ay += (double)ty;
     }
@@ -201,22 +202,30 @@ __device__ __forceinline__ void move_one(double2 *pos, int i, double fx, double 
     eterm[i] = fx * fx + fy * fy;
 }
 
-// FP32 mode: one wave per batch vertex sums its P partials (FP64, fixed tree order).
+// FP32 mode: one workgroup per batch vertex sums its P partials (FP64, fixed order:
+// strided per thread, then a fixed tree), so every partial row is read by 256 lanes at once.
 __global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part, int P, double2 *__restrict__ pos,
                                                int b0, int bc, double step, double *__restrict__ eterm) {
-    const int slot = (int)blockIdx.x * (WG / 64) + threadIdx.x / 64;
-    const int lane = threadIdx.x & 63;
+    __shared__ double2 red[WG / 64];
+    const int slot = (int)blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (slot >= bc) return;
     const double2 *row = part + (size_t)slot * P;
     double sx = 0.0, sy = 0.0;
-    for (int p = lane; p < P; p += 64) {
+    for (int p = tid; p < P; p += WG) {
         const double2 v = row[p];
         sx += v.x;
         sy += v.y;
     }
     sx = wave_sum(sx);
     sy = wave_sum(sy);
-    if (lane == 0) move_one(pos, b0 + slot, sx, sy, step, eterm);
+    if (lane == 0) red[w] = make_double2(sx, sy);
+    __syncthreads();
+    if (tid == 0) {
+        const double fx = (red[0].x + red[1].x) + (red[2].x + red[3].x);
+        const double fy = (red[0].y + red[1].y) + (red[2].y + red[3].y);
+        move_one(pos, b0 + slot, fx, fy, step, eterm);
+    }
 }
 
 // FP64 mode: prevCoordinates[i] += f per j block in block order (:771), then the move.
@@ -306,6 +315,9 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     double *eterm = g->eterm.as<double>();
     double *energy = g->energy.as<double>();
 
+    Prof prof;
+    prof.on = p->profile != 0;
+    prof.s = s;
     double STEP = step0;
     double E = 0.0, E0 = 1.79769313486231570815e+308;
     int loop = 0;
@@ -320,13 +332,18 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
             const int b0 = b * B;
             const int bc = (n - b0) < B ? (n - b0) : B;
             if (fp64) {
+                prof.begin(0);
                 hipLaunchKernelGGL(k_pairs64, dim3(G, (bc + WG - 1) / WG), dim3(WG), 0, s, pos, n, b0, bc, part, P,
                                    erp, eci, ew, p->weight, law);
+                prof.end();
+                prof.begin(1);
                 hipLaunchKernelGGL(k_move64, dim3((bc + WG - 1) / WG), dim3(WG), 0, s, part, P, pos, b0, bc, STEP,
                                    eterm);
+                prof.end();
             } else {
                 const int ipw = WG / JS;
                 const dim3 grid(G + (bc + 3) / 4, (bc + ipw - 1) / ipw);
+                prof.begin(0);
                 if (JS == 4)
                     hipLaunchKernelGGL(k_pairs32<4>, grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part, P, erp,
                                        eci, ew, p->weight, law);
@@ -336,12 +353,18 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
                 else
                     hipLaunchKernelGGL(k_pairs32<1>, grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part, P, erp,
                                        eci, ew, p->weight, law);
-                hipLaunchKernelGGL(k_move32, dim3((bc + 3) / 4), dim3(WG), 0, s, part, P, pos, b0, bc, STEP, eterm);
+                prof.end();
+                prof.begin(1);
+                hipLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, part, P, pos, b0, bc, STEP, eterm);
+                prof.end();
             }
         }
         (void)ncorr;
+        prof.begin(2);
         int rc = launch_energy(eterm, n, energy, p->precision, s);
+        prof.end();
         if (rc) return rc;
+        prof.flush();
         STEP = STEP * decay;  // :781 (0.999), :934 / :1007 (0.99)
         ++loop;
         if (converged) {
@@ -354,6 +377,7 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
         BL_HIP(hipMemcpyAsync(&E, energy, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     BL_HIP(hipStreamSynchronize(s));
+    prof.report(st);
     if (st) {
         st->energy = loop > 0 ? E : (law == LAW_FR ? 0.0 : 1.79769313486231570815e+308);
         st->iters_done = loop;

@@ -67,27 +67,21 @@ def _rgg_edges(pts: np.ndarray, r: float) -> np.ndarray:
 
 
 def rgg(n: int = 250_000, seed: int = 1, mean_degree: float = 8.0, return_points: bool = False):
-    """Random geometric graph in [0,1)^2; radius bisected so the mean degree is ~mean_degree.
+    """Random geometric graph in [0,1)^2 with mean degree mean_degree before the LCC cut.
 
-    The radius starts from sqrt(d/(pi n)) and is refined by bisection on the measured
-    mean degree (boundary vertices have fewer neighbours); the largest connected
-    component is kept.
+    The radius is the k-th smallest pair distance, k = mean_degree*n/2, taken from all
+    pairs within 1.2*sqrt(d/(pi n)) (boundary vertices have fewer neighbours, so the
+    radius lands slightly above the bulk estimate); the largest connected component is kept.
     """
     rng = np.random.default_rng(seed)
     pts = rng.random((n, 2))
     r0 = np.sqrt(mean_degree / (np.pi * n))
-    lo, hi = r0, r0 * 1.05
-    for _ in range(30):
-        mid = 0.5 * (lo + hi)
-        deg = 2.0 * len(_rgg_edges(pts, mid)) / n
-        if deg < mean_degree:
-            lo = mid
-        else:
-            hi = mid
-        if hi - lo < 1e-9:
-            break
-    r = hi
-    e = _canon(_rgg_edges(pts, r))
+    cand = _rgg_edges(pts, 1.2 * r0)
+    dist = np.sqrt(((pts[cand[:, 0]] - pts[cand[:, 1]]) ** 2).sum(1))
+    k = int(round(mean_degree * n / 2))
+    order = np.argsort(dist, kind="stable")[:k]
+    r = float(dist[order[-1]])
+    e = _canon(cand[order])
     n2, e2, keep = largest_component(n, e)
     if return_points:
         return n2, e2, pts[keep], r

@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""BatchLayout throughput on one MI355X.
+
+Workload (the layout benchmark's single-GPU configuration): random geometric graph of
+250,000 points uniform in the unit square (seed 1), radius set for mean degree 8, largest
+connected component kept (249,718 vertices, ~2.0 M CSR entries); initial coordinates
+uniform in [0,1)^2 (seed 0); algo 2 (cache-blocking minibatch), -batch 256, 100 iterations.
+
+A "step" is one layout iteration (every vertex updated once, ceil(n/256) batch steps).
+`value` = vertex-updates/s = n * steps / seconds of the timed region, positions resident in
+HBM before the region starts.  The energy of the FP32 run is checked against the
+reference-order FP64 path run on the same GPU from the same start (`energy` field).
+
+Also reported:
+  roofline      the dominant kernel (all-pairs repulsion, k_pairs32): 10 FP32 flops per
+                pair x pairs per launch / mean launch time from HIP events on the launch
+                stream, against the MI355X FP32 peak (157.3 TFLOP/s)
+  cpu_baseline  the reference built from its own sources (oracle/_ref/ref_fast) or, when
+                that binary is absent, the C oracle, timed on this host's cores for one
+                iteration of the same workload
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "vertex-updates/s at energy within 1% of FP64 ref; learned vs plain BH, 1 GPU"
+PEAK_FP32_TFLOPS = 157.3
+FLOPS_PER_PAIR = 10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, rp, ci, init, batch, threads):
+    """One iteration of the workload on the host cores (bounded sample)."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "ref_fast")
+    if os.path.exists(ref_bin):
+        from batchlayoutcode_amd import graphs
+
+        with tempfile.TemporaryDirectory() as td:
+            mtx = os.path.join(td, "rgg.mtx")
+            initf = os.path.join(td, "init.txt")
+            e = np.stack([np.repeat(np.arange(n), np.diff(rp)), ci], 1)
+            e = e[e[:, 0] < e[:, 1]]
+            graphs.write_mtx(mtx, n, e)
+            graphs.write_initf(initf, init)
+            t0 = time.time()
+            out = subprocess.run([ref_bin, "run", mtx, initf, "2", "1", str(batch), "1.2", "1.0", "0.01",
+                                  str(threads)], capture_output=True, text=True, cwd=td, check=True).stdout
+            wall = time.time() - t0
+        secs = None
+        for ln in out.splitlines():
+            if "Wall time required:" in ln:
+                secs = float(ln.split(":")[-1])
+        if secs is None:
+            secs = wall
+        return dict(value=n / secs, unit="vertex-updates/s", cores=threads, kind="reference",
+                    sample=f"1 iteration of algo 2 on the same {n}-vertex graph, -batch {batch}, "
+                           f"-threads {threads} (reference built from source with its Makefile flags); "
+                           f"{secs:.2f} s")
+    from oracle import pyoracle
+
+    t0 = time.time()
+    pyoracle.minibatch(rp, ci, init, 1, batch, threads=threads)
+    secs = time.time() - t0
+    return dict(value=n / secs, unit="vertex-updates/s", cores=threads, kind="port",
+                sample=f"1 iteration of algo 2 on the same {n}-vertex graph with the C oracle, "
+                       f"{threads} threads; {secs:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed layout iterations")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from batchlayoutcode_amd import api, graphs
+
+    t0 = time.time()
+    n, edges = graphs.rgg(250_000, seed=1)
+    rp, ci = graphs.csr_from_edges(n, edges)
+    init = graphs.uniform_init(n, 0)
+    log(f"graph: n={n} nnz={len(ci)} ({time.time() - t0:.1f} s); device: {api.device_info()}")
+
+    dev = torch.device("cuda:0")
+    d_init = torch.from_numpy(init).to(dev)
+    d_xy = torch.empty_like(d_init)
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+    K, W, B = args.steps, args.warmup, args.batch
+    with api.Graph(rp, ci) as g:
+        if W > 0:
+            d_xy.copy_(d_init)
+            g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=W, batch=B)
+        # timed region
+        d_xy.copy_(d_init)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        st = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        secs = ev0.elapsed_time(ev1) / 1e3
+        e32 = st.energy
+        log(f"timed: {K} iterations in {secs:.3f} s, E={e32:.6e}")
+        # profiled pass: per-launch HIP events on the same stream
+        d_xy.copy_(d_init)
+        torch.cuda.synchronize()
+        sp_prof = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B, profile=1)
+        pair_ms = sp_prof.kernel_ms[0] / max(1, sp_prof.kernel_launches[0])
+        move_ms = sp_prof.kernel_ms[1] / max(1, sp_prof.kernel_launches[1])
+        log(f"profiled: pair kernel {pair_ms*1e3:.2f} us/launch x {sp_prof.kernel_launches[0]}, "
+            f"move {move_ms*1e3:.2f} us/launch")
+        energy = dict(fp32=e32)
+        if not args.no_fp64_check:
+            d_xy.copy_(d_init)
+            torch.cuda.synchronize()
+            t1 = time.time()
+            s64 = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B, precision=api.PREC_FP64)
+            torch.cuda.synchronize()
+            rel = abs(e32 - s64.energy) / s64.energy
+            energy.update(fp64_gpu=s64.energy, rel_diff=rel, within_1pct=bool(rel <= 0.01),
+                          fp64_seconds=time.time() - t1)
+            log(f"fp64 check: E64={s64.energy:.6e} rel diff {rel:.2e}")
+
+    nb = (n + B - 1) // B
+    flops_per_launch = FLOPS_PER_PAIR * n * (n - 1) / nb
+    achieved = flops_per_launch / (pair_ms / 1e3) / 1e12
+    roofline = dict(bound="valu", kernel="k_pairs32", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None, mean_launch_us=pair_ms * 1e3,
+                    launches=sp_prof.kernel_launches[0], flops_per_launch=flops_per_launch)
+    out = dict(metric=METRIC, value=n * K / secs, unit="vertex-updates/s", n_gpus=1, steps=K, warmup=W,
+               ms_per_step=secs * 1e3 / K, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32",
+               data="synthetic",
+               config=dict(workload="rgg250k-algo2", graph="random geometric, 250,000 points, seed 1, mean degree 8, "
+                           "largest component", n=n, nnz=int(len(ci)), batch=B, iterations=K, init="uniform seed 0",
+                           parallelism="none (1 GPU)"),
+               roofline=roofline, energy=energy)
+    if not args.no_cpu:
+        threads = os.cpu_count() or 1
+        log(f"cpu baseline on {threads} threads ...")
+        out["cpu_baseline"] = cpu_baseline(n, rp, ci, init, B, threads)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} vertex-updates/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,7 +65,8 @@ typedef struct {
     double step0;       /* STEP of the first iteration (1.0 as in the reference; a resumed
                            or teacher-forced run passes the STEP it continues from) */
     int32_t precision;  /* BL_PREC_* */
-    int32_t reserved;
+    int32_t profile;    /* nonzero: time every launch with HIP events on the run's stream and
+                           report per-kernel-class totals in bl_stats (adds a sync per iteration) */
 } bl_params;
 
 typedef struct {
@@ -74,6 +75,11 @@ typedef struct {
     int32_t status;       /* BL_OK, or the first error met */
     double seconds;       /* wall time of the layout loop (the reference's end - start) */
     double step_final;    /* STEP after the last iteration */
+    /* with bl_params.profile: GPU time (ms) and launch count per kernel class:
+       [0] pair forces / tree traversal, [1] reduce+move / attraction+move,
+       [2] energy reduction, [3] tree build (bounding box, keys, sort, nodes, centroids) */
+    double kernel_ms[4];
+    int32_t kernel_launches[4];
 } bl_stats;
 
 /* Fills *p with the reference CLI defaults. */

@@ -1,0 +1,96 @@
+"""C ABI checks that need no GPU: every declared symbol is exported, and the host I/O
+entry points (bl_mtx_read, bl_initf_read, bl_output_write) reproduce the reference's
+file handling (IO.h:60-155, algorithms.cpp:98-119, :1244-1256)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden, golden_files, golden_input
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "batchlayout.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(bl_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_every_declared_symbol_is_exported(bl):
+    syms = declared_symbols()
+    assert len(syms) >= 14
+    L = bl.lib()
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("fx", golden_files("csr_*.npz"))
+def test_mtx_read_matches_reference_csr(bl, fx):
+    d = golden(fx)
+    rp, ci = bl.mtx_read(golden_input(fx[4:-4] + ".mtx"))
+    assert np.array_equal(rp, d["rowptr"]) and np.array_equal(ci, d["colids"])
+
+
+def test_mtx_read_matches_oracle_on_random_files(bl, oracle, tmp_path):
+    rng = np.random.default_rng(3)
+    for t in range(20):
+        n = int(rng.integers(2, 60))
+        m = int(rng.integers(0, 3 * n))
+        sym = bool(rng.integers(0, 2))
+        e = rng.integers(1, n + 1, size=(m, 2))
+        head = "%%MatrixMarket matrix coordinate pattern " + ("symmetric" if sym else "general")
+        p = tmp_path / f"r{t}.mtx"
+        with open(p, "w") as f:
+            f.write(head + "\n% comment\n" + f"{n} {n} {m}\n")
+            for a, b in e:
+                f.write(f"{a} {b}\n")
+        r1 = bl.mtx_read(str(p))
+        r2 = oracle.mtx_read(str(p))
+        assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+
+
+def test_mtx_read_errors(bl, tmp_path):
+    cases = {
+        "short.mtx": ("%%MatrixMarket matrix coordinate pattern symmetric\n4 4 3\n2 1\n3 2\n", -3),
+        "range.mtx": ("%%MatrixMarket matrix coordinate pattern general\n3 3 1\n4 1\n", -3),
+        "rect.mtx": ("%%MatrixMarket matrix coordinate pattern general\n3 4 1\n1 2\n", -3),
+    }
+    for name, (text, code) in cases.items():
+        p = tmp_path / name
+        p.write_text(text)
+        with pytest.raises(bl.BLError) as ei:
+            bl.mtx_read(str(p))
+        assert ei.value.code == code
+    with pytest.raises(bl.BLError) as ei:
+        bl.mtx_read(str(tmp_path / "does_not_exist.mtx"))
+    assert ei.value.code == -2
+
+
+def test_initf_read(bl, tmp_path):
+    p = tmp_path / "init.txt"
+    xy = np.random.default_rng(1).random((5, 2))
+    with open(p, "w") as f:
+        for i, (x, y) in enumerate(xy):
+            f.write("%.17g %.17g %d\n" % (x, y, i + 1))
+            if i == 1:
+                f.write("\n")  # a line that does not parse repeats the previous coordinate
+    got, k = bl.initf_read(str(p), 7)
+    assert k == 6
+    assert np.array_equal(got[:2], xy[:2]) and np.array_equal(got[2], xy[1]) and np.array_equal(got[3:6], xy[2:])
+    assert np.array_equal(got[6], [0.0, 0.0])
+    got, k = bl.initf_read(str(p), 3)  # at most n lines are read
+    assert k == 3
+
+
+@pytest.mark.parametrize("fx", golden_files("out_*.txt"))
+def test_output_write_is_byte_identical_to_reference(bl, fx, tmp_path):
+    case, k = fx[4:-4].rsplit("_k", 1)
+    d = golden(f"traj_{case}.npz")
+    kk = list(d["ks"]).index(int(k))
+    out = tmp_path / "o.txt"
+    bl.output_write(str(out), d["xy"][kk])
+    assert out.read_bytes() == open(os.path.join(ROOT, "tests", "golden", fx), "rb").read()
+
+
+def test_params_default_is_reference_cli_default(bl):
+    p = bl.params()
+    assert (p.algo, p.iters, p.batch, p.bht, p.engt, p.weight, p.step0) == (2, 600, 256, 1.2, 0.01, 1.0, 1.0)

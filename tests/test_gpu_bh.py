@@ -1,0 +1,120 @@
+"""GPU parity of the Barnes-Hut path (algo 4): Morton scales, sort permutation and
+quadtree bit-exact against the reference's own dumps; forces bit-exact (FP64 mode) or
+within FP32 tolerance; algo 4 layouts bit-exact (FP64) / teacher-forced (FP32)."""
+import numpy as np
+import pytest
+
+from conftest import diameter, golden, golden_files, golden_input
+from treeutil import canonical_from_child_array, canonical_from_reference
+
+pytestmark = pytest.mark.gpu
+BH = golden_files("bh_*.npz")
+
+
+@pytest.mark.parametrize("fx", BH)
+def test_morton_scales_bit_exact(bl, fx):
+    d = golden(fx)
+    print(f"[case] morton {fx}", flush=True)
+    xs, ys, mm = bl.bh_morton(d["pts"])
+    assert mm == d["maxmin"]
+    assert np.array_equal(xs, d["scales_strict"][:, 0]) and np.array_equal(ys, d["scales_strict"][:, 1])
+
+
+@pytest.mark.parametrize("fx", BH)
+def test_permutation_and_tree_bit_exact(bl, fx):
+    d = golden(fx)
+    print(f"[case] tree {fx}", flush=True)
+    t = bl.bh_tree(d["pts"])
+    assert np.array_equal(t["perm"], d["perm"])
+    got = canonical_from_child_array(t["child"], t["diam"], t["cx"], t["cy"], t["root"])
+    assert got == canonical_from_reference(d)
+
+
+@pytest.mark.parametrize("fx", BH)
+def test_forces(bl, fx):
+    d = golden(fx)
+    print(f"[case] forces {fx}", flush=True)
+    f64 = bl.bh_forces(d["pts"], float(d["theta"]), precision=bl.PREC_FP64)
+    assert np.array_equal(f64, d["force_strict"])
+    f32 = bl.bh_forces(d["pts"], float(d["theta"]), precision=bl.PREC_FP32)
+    err = np.abs(f32 - d["force_strict"]).max()
+    scale = np.abs(d["force_strict"]).max()
+    print(f"{fx}: FP32 force max abs err {err:.2e} (max |F| {scale:.1f})")
+    assert err <= 2e-5 * max(1.0, scale)
+
+
+def test_forces_at_query_points_match_oracle(bl, oracle):
+    rng = np.random.default_rng(4)
+    pts = rng.random((3000, 2))
+    q = rng.random((777, 2)) * 1.4 - 0.2
+    for theta in (0.4, 1.2):
+        ref = oracle.bh_forces(pts, theta, q)
+        got = bl.bh_forces(pts, theta, q, precision=bl.PREC_FP64)
+        assert np.array_equal(got, ref)
+
+
+def test_duplicate_points_are_an_error(bl):
+    pts = np.random.default_rng(2).random((100, 2))
+    pts[17] = pts[3]
+    with pytest.raises(bl.BLError) as ei:
+        bl.bh_tree(pts)
+    assert ei.value.code == -5
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_tiny_point_sets(bl, oracle, n):
+    pts = np.random.default_rng(n).random((n, 2))
+    ref = oracle.bh_forces(pts, 1.2)
+    got = bl.bh_forces(pts, 1.2, precision=bl.PREC_FP64)
+    assert np.array_equal(got, ref)
+
+
+A4 = golden_files("traj_a4_*.npz")
+
+
+@pytest.mark.parametrize("fx", A4)
+def test_algo4_fp64_matches_strict_reference(bl, fx):
+    d = golden(fx)
+    rp, ci = bl.mtx_read(golden_input(str(d["input"])))
+    with bl.Graph(rp, ci) as g:
+        for kk, k in enumerate(d["ks"]):
+            print(f"[case] {fx} k={k} fp64", flush=True)
+            xy, st = g.run(d["init"], algo=4, iters=int(k), batch=int(d["batch"]), bht=float(d["bht"]),
+                           precision=bl.PREC_FP64)
+            assert np.array_equal(xy, d["xy_strict"][kk]), (fx, k, np.abs(xy - d["xy_strict"][kk]).max())
+            assert st.energy == d["E_strict"][kk]
+
+
+@pytest.mark.parametrize("fx", A4)
+def test_algo4_fp32_first_and_teacher_forced(bl, fx):
+    d = golden(fx)
+    rp, ci = bl.mtx_read(golden_input(str(d["input"])))
+    ks = list(d["ks"])
+    kw = dict(algo=4, iters=1, batch=int(d["batch"]), bht=float(d["bht"]), precision=bl.PREC_FP32)
+    with bl.Graph(rp, ci) as g:
+        xy, st = g.run(d["init"], **kw)
+        err = np.abs(xy - d["xy_strict"][0]).max() / diameter(d["xy_strict"][0])
+        print(f"{fx}: iteration 1 max|dx|/diam = {err:.3e}")
+        assert err <= 1e-4
+        for kk in range(1, len(ks)):
+            if ks[kk] != ks[kk - 1] + 1:
+                continue
+            xy, st = g.run(d["xy_strict"][kk - 1], step0=0.999 ** ks[kk - 1], **kw)
+            ref = d["xy_strict"][kk]
+            err = np.abs(xy - ref).max() / diameter(ref)
+            print(f"{fx}: {ks[kk-1]}->{ks[kk]} max|dx|/diam = {err:.3e}")
+            assert err <= 1e-4
+
+
+def test_algo4_final_energy_600_iterations(bl):
+    band = golden("band_a4_grid64_b256_600.npz")
+    e_ref = float(band["E"][0])
+    tol = max(0.01, (band["E"].max() - band["E"].min()) / e_ref)
+    init = golden("traj_a4_grid64_b256.npz")["init"]
+    rp, ci = bl.mtx_read(golden_input("grid64x64.mtx"))
+    with bl.Graph(rp, ci) as g:
+        print("[case] grid64 algo 4 600 iterations", flush=True)
+        xy, st = g.run(init, algo=4, iters=600, batch=256, precision=bl.PREC_FP32)
+    rel = abs(st.energy - e_ref) / e_ref
+    print(f"algo 4: E={st.energy:.6e} ref={e_ref:.6e} rel={rel:.3e} tol={tol:.3e} ({st.seconds*1e3:.1f} ms)")
+    assert rel <= tol
