@@ -1,0 +1,280 @@
+// Host mirror of the reference's `algorithms` class on libbatchlayout.so (see algorithms.hpp).
+#include "algorithms.hpp"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <iostream>
+#include <sstream>
+#include <vector>
+
+namespace bl {
+
+static const double kMAXMIN = 3.0;                    // algorithms.h:43
+static const double kPI = 3.14159265358979323846;     // algorithms.h:45
+
+static double wtime() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int SetInputMatricesAsCSR(CSRGraph &g, const std::string &inputfile) {
+    std::cout << "Reading input matrices in text (ascii)... " << std::endl;
+    std::cout << "Input File Directory:" << inputfile << std::endl;
+    // the reference reports "Matrix is symmetric" once per header line that says so
+    if (FILE *f = fopen(inputfile.c_str(), "r")) {
+        char line[256];
+        int c;
+        while ((c = fgetc(f)) == '%') {
+            if (!fgets(line, sizeof(line), f)) break;
+            if (strstr(line, "symmetric")) std::cout << "Matrix is symmetric" << std::endl;
+        }
+        fclose(f);
+    }
+    const double t0 = wtime();
+    int32_t n = 0;
+    int64_t nnz = 0;
+    int32_t *rp = nullptr, *ci = nullptr;
+    const int rc = bl_mtx_read(inputfile.c_str(), &n, &nnz, &rp, &ci);
+    if (rc != BL_OK) return rc;
+    printf("Converting matrix data from ASCII to COO format: %.16g seconds\n", wtime() - t0);
+    printf("Input Matrix: Rows = %d, Columns= %d, nnz = %d\n", n, n, (int)nnz);
+    fflush(stdout);
+    std::cout << "Converting to csc ... " << std::endl << std::endl;
+    g.rows = n;
+    g.nnz = nnz;
+    g.rowptr.assign(rp, rp + n + 1);
+    g.colids.assign(ci, ci + nnz);
+    bl_free(rp);
+    bl_free(ci);
+    return BL_OK;
+}
+
+algorithms::algorithms(CSRGraph &A_csr, std::string input, std::string outputd, int init_, double weight,
+                       double th, std::string ifile)
+    : graph(A_csr), W(weight), threshold(th), init(init_), filename(input), outputdir(outputd), initfile(ifile) {
+    nCoordinates.assign(2 * (size_t)graph.rows, 0.0);
+    status = bl_graph_create(graph.rows, graph.nnz, graph.rowptr.data(), graph.colids.data(), nullptr, &g_);
+    if (status != BL_OK) {
+        fprintf(stderr, "batchlayout: cannot create the device graph (status %d)\n", status);
+        g_ = nullptr;
+    }
+}
+
+algorithms::~algorithms() {
+    if (g_) bl_graph_destroy(g_);
+}
+
+void algorithms::randInit() {
+    for (int32_t i = 0; i < graph.rows; i++) {
+        double x, y;
+        do {
+            x = -1.0 + 2.0 * rand() / (RAND_MAX + 1.0);
+            y = -1.0 + 2.0 * rand() / (RAND_MAX + 1.0);
+        } while (x * x + y * y > 1.0);
+        nCoordinates[2 * i] = x * kMAXMIN;
+        nCoordinates[2 * i + 1] = y * kMAXMIN;
+    }
+}
+
+// Greedy depth-first placement, algorithms.cpp:29-97: every unvisited neighbour of a
+// popped vertex goes on a circle of radius 0.1 around it (angles in steps of 360/deg),
+// shifted by one rand() draw added to both coordinates; the layout is then scaled to
+// 2*MAXMIN over its larger extent.  Rows that are empty or last walk every CSR entry from
+// rowptr[parent] to the end, as the reference does.
+void algorithms::initDFS() {
+    const int32_t n = graph.rows;
+    std::vector<int> visited(n, 0);
+    std::vector<int> stack;
+    double minX = 1.0, minY = 1.0, maxX = 1.0, maxY = 1.0;
+    const double radi = 0.1;
+    stack.push_back(0);
+    visited[0] = 1;
+    nCoordinates[0] = 1.0;
+    nCoordinates[1] = 1.0;
+    auto place = [&](int32_t parent, int64_t from, int64_t to) {
+        const double deg = 360.0 / (double)(to - from);
+        double degree = 0;
+        for (int64_t k = from; k < to; k++) {
+            const int32_t c = graph.colids[k];
+            if (visited[c] == 0) {
+                const double bx = nCoordinates[2 * parent] + radi * cos(kPI * (degree) / 180.0);
+                const double by = nCoordinates[2 * parent + 1] + radi * sin(kPI * (degree) / 180.0);
+                const float r = static_cast<float>(rand()) / static_cast<float>(RAND_MAX);
+                nCoordinates[2 * c] = bx + r;
+                nCoordinates[2 * c + 1] = by + r;
+                visited[c] = 1;
+                stack.push_back(c);
+                degree += deg;
+                const double x = nCoordinates[2 * c], y = nCoordinates[2 * c + 1];
+                if (minX > x) minX = x;
+                else if (maxX < x) maxX = x;
+                if (minY > y) minY = y;
+                else if (maxY < y) maxY = y;
+            }
+        }
+    };
+    while (!stack.empty()) {
+        const int parent = stack.back();
+        stack.pop_back();
+        if (parent < n - 1 && (graph.rowptr[parent + 1] - graph.rowptr[parent]) > 0) {
+            place(parent, graph.rowptr[parent], graph.rowptr[parent + 1]);
+        } else {
+            if ((graph.nnz - graph.rowptr[parent]) <= 0) continue;
+            place(parent, graph.rowptr[parent], graph.nnz);
+        }
+    }
+    const double scalefactor = 2.0 * kMAXMIN / std::max(maxX - minX, maxY - minY);
+    for (int32_t i = 0; i < n; i++) {
+        nCoordinates[2 * i] = nCoordinates[2 * i] * scalefactor;
+        nCoordinates[2 * i + 1] = nCoordinates[2 * i + 1] * scalefactor;
+    }
+}
+
+void algorithms::fileInitialization() {
+    int32_t got = 0;
+    if (bl_initf_read(initfile.c_str(), graph.rows, nCoordinates.data(), &got) != BL_OK) {
+        std::cout << "ERROR in input coordinates file!\n" << std::endl;
+        exit(1);
+    }
+    if (got < graph.rows)
+        fprintf(stderr, "batchlayout: %s has %d coordinate lines for %d vertices; the rest start at (0,0)\n",
+                initfile.c_str(), got, graph.rows);
+}
+
+void algorithms::initialize() {
+    if (init == 0) initDFS();
+    else if (init == 2) fileInitialization();
+    else randInit();
+}
+
+int algorithms::gpu_run(int algo, int iters, int batch, double bht, int converged, double step0, bl_stats *st) {
+    memset(st, 0, sizeof(*st));
+    if (!g_) return status;
+    bl_params p;
+    bl_params_default(&p);
+    p.algo = algo;
+    p.iters = iters;
+    p.batch = batch;
+    p.bht = bht;
+    p.converged = converged;
+    p.engt = threshold;
+    p.weight = W;
+    p.step0 = step0;
+    p.precision = precision;
+    status = bl_run(g_, nCoordinates.data(), &p, st);
+    if (status != BL_OK) fprintf(stderr, "batchlayout: layout failed with status %d\n", status);
+    return status;
+}
+
+std::vector<double> algorithms::cacheBlockingminiBatchForceDirectedAlgorithm(int ITERATIONS, int, int BATCHSIZE,
+                                                                             int flag) {
+    const double start = wtime();
+    double step0 = 1.0;
+    if (flag == 0) initialize();
+    else step0 = pow(0.999, 4 * ITERATIONS);  // :736-737
+    bl_stats st;
+    gpu_run(BL_ALGO_CACHE, ITERATIONS, BATCHSIZE, 1.2, 0, step0, &st);
+    const double end = wtime();
+    if (flag == 0) {
+        std::cout << "Cache Blocking Minibatch Size:" << BATCHSIZE << std::endl;
+        std::cout << "Cache Blocking Minbatch Energy:" << st.energy << std::endl;
+        std::cout << "Cache Blocking Minibatch Parallel Wall time required:" << end - start << std::endl;
+        writeToFile("CACHEMINB" + std::to_string(BATCHSIZE) + "PARAOUT" + std::to_string(st.iters_done));
+    }
+    return {st.energy, end - start};
+}
+
+std::vector<double> algorithms::cacheBlockingminiBatchForceDirectedAlgorithmConverged(int ITERATIONS, int,
+                                                                                      int BATCHSIZE, int flag) {
+    const double start = wtime();
+    double step0 = 1.0;
+    if (flag == 0) initialize();
+    else step0 = pow(0.999, 4 * ITERATIONS);
+    bl_stats st;
+    gpu_run(BL_ALGO_CACHE, ITERATIONS, BATCHSIZE, 1.2, 1, step0, &st);
+    const double end = wtime();
+    if (flag == 0) {
+        std::cout << "Cache Blocking (converged) Minibatch Size:" << BATCHSIZE << std::endl;
+        std::cout << "Cache Blocking (converged)  Minbatch Energy:" << st.energy << std::endl;
+        std::cout << "Cache Blocking (converged) Minibatch Parallel Wall time required:" << end - start << std::endl;
+        writeToFile("CACHEMINB" + std::to_string(BATCHSIZE) + "PARAOUT" + std::to_string(st.iters_done));
+    }
+    return {st.energy, end - start};
+}
+
+std::vector<double> algorithms::LinLogcacheBlockingminiBatchForceDirectedAlgorithm(int ITERATIONS, int,
+                                                                                   int BATCHSIZE) {
+    const double start = wtime();
+    initialize();
+    bl_stats st;
+    gpu_run(BL_ALGO_LINLOG, ITERATIONS, BATCHSIZE, 1.2, 0, 1.0, &st);
+    const double end = wtime();
+    std::cout << "LinLog Batch Size:" << BATCHSIZE << std::endl;
+    std::cout << "LinLog Minbatch Energy:" << st.energy << std::endl;
+    std::cout << "LinLog Minibatch Parallel Wall time required:" << end - start << std::endl;
+    writeToFile("LLCACHEMINB" + std::to_string(BATCHSIZE) + "PARAOUT" + std::to_string(ITERATIONS));
+    return {st.energy, end - start};
+}
+
+std::vector<double> algorithms::FAcacheBlockingminiBatchForceDirectedAlgorithm(int ITERATIONS, int, int BATCHSIZE) {
+    const double start = wtime();
+    initialize();
+    bl_stats st;
+    gpu_run(BL_ALGO_FA, ITERATIONS, BATCHSIZE, 1.2, 0, 1.0, &st);
+    const double end = wtime();
+    std::cout << "FA Batch Size:" << BATCHSIZE << std::endl;
+    std::cout << "FA Minbatch Energy:" << st.energy << std::endl;
+    std::cout << "FA Minibatch Parallel Wall time required:" << end - start << std::endl;
+    writeToFile("FACACHEMINB" + std::to_string(BATCHSIZE) + "PARAOUT" + std::to_string(ITERATIONS));
+    return {st.energy, end - start};
+}
+
+std::vector<double> algorithms::BarnesHutApproximation(int ITERATIONS, int, int BATCHSIZE, double TH, int flag) {
+    const double start = wtime();
+    if (flag == 0) initialize();
+    bl_stats st;
+    gpu_run(BL_ALGO_BH, ITERATIONS, BATCHSIZE, TH, 0, 1.0, &st);
+    const double end = wtime();
+    std::vector<double> result;
+    if (flag == 0) {  // the flag == 1 call returns an empty vector, as the reference (:1074-1081)
+        std::cout << "Barnes-Hut Minibatch Size:" << BATCHSIZE << std::endl;
+        std::cout << "Barnes-Hut Minbatch Energy:" << st.energy << std::endl;
+        result.push_back(st.energy);
+        std::cout << "Barnes-Hut Minibatch Parallel Wall time required:" << end - start << std::endl;
+        result.push_back(end - start);
+        writeToFile("BHMINB" + std::to_string(BATCHSIZE) + "PARAOUT" + std::to_string(ITERATIONS));
+    }
+    return result;
+}
+
+std::vector<double> algorithms::approxCacheBlockBH(int ITERATIONS, int NUMOFTHREADS, int BATCHSIZE) {
+    const int approxITER = (int)(ITERATIONS * 0.8);
+    const double start = wtime();
+    initialize();
+    BarnesHutApproximation(approxITER, NUMOFTHREADS, BATCHSIZE, 1.2, 1);
+    std::vector<double> result =
+        cacheBlockingminiBatchForceDirectedAlgorithm(ITERATIONS - approxITER, NUMOFTHREADS, BATCHSIZE, 1);
+    const double end = wtime();
+    result[1] = end - start;
+    std::cout << "80%% - 20%% BH - CB" << std::endl;
+    std::cout << "BH-CACHE Approximation Energy:" << result[0] << std::endl;
+    std::cout << "BH-CACHE Approximation Wall time required:" << end - start << std::endl;
+    writeToFile("BCAPPROX" + std::to_string(BATCHSIZE) + "PARAOUT" + std::to_string(ITERATIONS));
+    return result;
+}
+
+void algorithms::writeToFile(const std::string &f) {
+    std::stringstream data(filename);
+    std::string lasttok;
+    while (std::getline(data, lasttok, '/')) {
+    }
+    filename = outputdir + lasttok + f + ".txt";  // the reference keeps the composed name
+    std::cout << "Creating output file in following directory:" << filename << std::endl;
+    if (bl_output_write(filename.c_str(), graph.rows, nCoordinates.data()) != BL_OK)
+        fprintf(stderr, "batchlayout: cannot write %s\n", filename.c_str());
+}
+
+}  // namespace bl

@@ -21,6 +21,7 @@ import argparse
 import os
 import shutil
 import subprocess
+import tempfile
 import sys
 
 import numpy as np
@@ -202,6 +203,22 @@ def main():
             np.savez_compressed(os.path.join(HERE, f"band_{case}.npz"), E=np.array(es),
                                 E_strict=np.array(es_strict), xy_final=final)
             print("band", case, es, es_strict)
+
+    # --- the reference CLI itself: stdout transcript, -init 0/1 placements --------------
+    cli = os.path.join(ROOT, "oracle", "_ref", "BatchLayout")
+    with tempfile.TemporaryDirectory() as td:
+        graphs.write_initf(os.path.join(td, "init.txt"), graphs.uniform_init(10, 0))
+        out = sh([cli, "-input", os.path.join(INP, "test.mtx"), "-output", td + "/", "-initf", "init.txt",
+                  "-iter", "3", "-threads", "2", "-algo", "2"], cwd=td)
+        with open(os.path.join(HERE, "cli_stdout_a2_test.txt"), "w") as f:
+            f.write(out.replace(td + "/", "<OUT>/").replace(os.path.join(INP, "test.mtx"), "<INPUT>"))
+        shutil.copy(os.path.join(td, "Results.txt"), os.path.join(HERE, "cli_results_a2_test.txt"))
+        for name in ("test.mtx", "jagmesh1.mtx", "grid8x8.mtx"):
+            for init in (0, 1):
+                sh([cli, "-input", os.path.join(INP, name), "-output", td + "/", "-init", str(init), "-iter", "0",
+                    "-threads", "1", "-algo", "2"], cwd=td)
+                shutil.copy(os.path.join(td, f"{name}CACHEMINB256PARAOUT0.txt"),
+                            os.path.join(HERE, f"cli_init{init}_{name[:-4]}.txt"))
 
     # --- Barnes-Hut structures on point clouds ---------------------------------------
     rng = np.random.default_rng(11)
