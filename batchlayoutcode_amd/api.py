@@ -18,7 +18,7 @@ BL_OK = 0
 STATUS = {0: "BL_OK", -1: "BL_E_ARG", -2: "BL_E_IO", -3: "BL_E_FORMAT", -4: "BL_E_DEVICE",
           -5: "BL_E_DUPKEY", -6: "BL_E_NOMEM", -7: "BL_E_GRAPH"}
 ALGO_CACHE, ALGO_LINLOG, ALGO_BH, ALGO_BHCACHE, ALGO_FA = 2, 3, 4, 6, 8
-PREC_FP32, PREC_FP64 = 0, 1
+PREC_MIXED, PREC_FP64, PREC_FP32 = 0, 1, 2
 BATCHED_MAX_N = 8192
 
 

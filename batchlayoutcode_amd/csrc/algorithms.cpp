@@ -145,6 +145,9 @@ void algorithms::fileInitialization() {
 }
 
 void algorithms::initialize() {
+    // the reference draws its first rand() values here, from glibc's default seed; the HIP
+    // runtime may have drawn some already in this process, so restart the sequence
+    if (init != 2) srand(1);
     if (init == 0) initDFS();
     else if (init == 2) fileInitialization();
     else randInit();

@@ -30,7 +30,7 @@ class algorithms {
     double W, threshold;
     int init;
     std::string filename, outputdir, initfile;
-    int precision = BL_PREC_FP32;      // GPU pair arithmetic (build extension)
+    int precision = BL_PREC_MIXED;     // GPU pair arithmetic (build extension)
     int status = BL_OK;                // last GPU status (the reference has none)
 
     // algorithms.cpp:3-15

@@ -84,7 +84,7 @@ extern "C" int bl_params_default(bl_params *p) {
     p->engt = 0.01;
     p->weight = 1.0;
     p->step0 = 1.0;
-    p->precision = BL_PREC_FP32;
+    p->precision = BL_PREC_MIXED;
     return BL_OK;
 }
 
@@ -194,7 +194,8 @@ extern "C" int bl_graph_size(const bl_graph *g, int32_t *n, int64_t *nnz) {
 static int validate_params(const bl_params *p) {
     if (!p) return BL_E_ARG;
     if (p->batch <= 0 || p->iters < 0) return BL_E_ARG;
-    if (p->precision != BL_PREC_FP32 && p->precision != BL_PREC_FP64) return BL_E_ARG;
+    if (p->precision != BL_PREC_MIXED && p->precision != BL_PREC_FP64 && p->precision != BL_PREC_FP32)
+        return BL_E_ARG;
     switch (p->algo) {
         case BL_ALGO_CACHE: case BL_ALGO_LINLOG: case BL_ALGO_BH: case BL_ALGO_BHCACHE: case BL_ALGO_FA:
             return BL_OK;

@@ -8,8 +8,9 @@
 // Results.txt row.  Layouts run on the GPU through libbatchlayout.so.
 //
 // Build-only flags use a prefix the reference never parses:
-//   --gpu-precision fp32|fp64   pair arithmetic (default fp32; fp64 follows the reference's
-//                               FP64 operation order)
+//   --gpu-precision mixed|fp32|fp64   pair arithmetic (default mixed: FP64 with an FP32-seeded
+//                               reciprocal; fp32: FP32 |d|^2 and reciprocal; fp64: the
+//                               reference's FP64 operation order)
 // Algorithms 0, 1, 5 and 7 (sequential / naive / greedy / algebraic baselines of the
 // reference) are not provided by this build; they exit with status 1 and a message.
 #include <math.h>
@@ -49,7 +50,7 @@ static void helpmessage() {
     printf("It indicates if energy value is improved less than this percenrages then optimization will stop and "
            "return layout. If this value is set, then number of iteration will be overwritten.\n");
     printf("-weight <float>, a real number to include edge weight.\n");
-    printf("--gpu-precision <fp32|fp64>, pair arithmetic on the GPU (build extension).\n");
+    printf("--gpu-precision <mixed|fp32|fp64>, pair arithmetic on the GPU (build extension).\n");
     printf("-h, show help message.\n");
     printf("default: -weight 1.0 -bht 1.2 -engt 0.01 -batch 256 -iter 600 -threads MAX -algo 2 -init 0\n\n");
 }
@@ -58,7 +59,7 @@ int main(int argc, char *argv[]) {
     double energyThreshold = 0.01, bhThreshold = 1.2, weight = 1.0;
     const int maxThreads = (int)std::max(1u, std::thread::hardware_concurrency());
     int init = 0, batchsize = 256, iterations = 600, numberOfThreads = maxThreads, algoOption = 2;
-    int precision = BL_PREC_FP32;
+    int precision = BL_PREC_MIXED;
     std::string inputfile, initfile, outputfile, algoname = "CACHE", initname = "GREEDY";
     for (int p = 0; p < argc; p++) {
         const bool has = p + 1 < argc;
@@ -82,8 +83,11 @@ int main(int argc, char *argv[]) {
             if (energyThreshold < 0.0 || energyThreshold > 1.0) energyThreshold = 0.01;
         }
         if (strcmp(argv[p], "-weight") == 0 && has) weight = atof(argv[p + 1]);
-        if (strcmp(argv[p], "--gpu-precision") == 0 && has)
-            precision = strcmp(argv[p + 1], "fp64") == 0 ? BL_PREC_FP64 : BL_PREC_FP32;
+        if (strcmp(argv[p], "--gpu-precision") == 0 && has) {
+            if (strcmp(argv[p + 1], "fp64") == 0) precision = BL_PREC_FP64;
+            else if (strcmp(argv[p + 1], "fp32") == 0) precision = BL_PREC_FP32;
+            else precision = BL_PREC_MIXED;
+        }
         if (strcmp(argv[p], "-h") == 0) {
             helpmessage();
             exit(1);

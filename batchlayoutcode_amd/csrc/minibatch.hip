@@ -11,25 +11,31 @@
 // where N(i) is the set of column ids the reference's merge pointer meets (:758-762)
 // and A is the attraction law (W|d| d, W d, or log2(1+W|d|) d).
 //
-// Two kernels per batch step:
-//   k_pairs32 / k_pairs64  all-pairs repulsion for the batch against all n vertices,
-//                          j split into chunks over workgroups, one partial per chunk
-//   k_move32  / k_move64   sum of the partials per vertex, unit-step move, |F|^2
-// and one energy reduction per iteration.
-//
-// FP32 mode (the fast path): each lane owns one batch vertex; the j loop is
-// wave-uniform, so x_j arrives through scalar loads and every pair costs
-// 2 FP64 subtracts + 2 converts, 2 FMA for |d|^2, one v_rcp_f32 and 2 FMA.  Sums run in
-// FP32 over 64-j tiles and in FP64 across tiles and chunks.  Neighbours are not
-// branched on in the dense loop: one extra workgroup per four batch vertices adds,
-// in FP64, the attraction of every neighbour and removes the repulsion term the
-// dense loop counted for it (the exact FP64 value of the FP32 product it added).
+// FP32 mode (the fast path).  Per pair: the FP64 difference d = x_j - x_i, |d|^2 and
+// 1/|d|^2 in FP32 (v_rcp_f32), and the term -d/|d|^2 accumulated in FP64 (one FMA per
+// component with the FP64 difference).  The dense loop runs over all j; neighbours are not
+// branched on: a correction adds, in FP64, each neighbour's attraction and removes the
+// repulsion product the dense loop accumulated for it (the same operands, so it cancels).
+//   * k_iter32 (default): one persistent launch per iteration.  Worker workgroups own a j
+//     chunk held in LDS for the whole iteration and compute, batch after batch, the chunk
+//     partial of every batch vertex; reducer workgroups form each batch vertex's correction,
+//     sum its partials in a fixed order and move it.  Steps are ordered by striped atomic
+//     counters: a worker only waits for the previous batch's moves when its own chunk holds
+//     those vertices (and then only for that part of the chunk, which it processes last),
+//     so the reduction and move of batch b overlap the bulk of batch b+1.  Every wait has a
+//     deadline; on expiry the launch sets an error flag and all workgroups leave.
+//   * k_pairs32 + k_move32: two launches per batch step, used when the cooperative launch
+//     does not fit (very large n or batch).
 //
 // FP64 mode (reference-grade): the reference's own operations, in its order.  The
 // j range is cut into the reference's 512-wide blocks (blocky, :717); each block's
 // partial is summed in j order with the merge-pointer branch, and the partials are
 // added in block order, so the result reproduces the strict-IEEE reference build.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
 
 #include "bl_internal.h"
 
@@ -41,6 +47,7 @@ constexpr int WG = 256;
 // 0 * (1/kTiny) = 0, as the reference's `dist > 0` test skips them.
 constexpr float kTiny = 1e-30f;
 constexpr int kBlockY = 512;  // the reference's j block, algorithms.cpp:717
+constexpr int UNROLL = 16;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -63,28 +70,248 @@ __device__ __forceinline__ void attract64(int law, double W, double dx, double d
 }
 
 // Read-only view of the positions in the constant address space: with a wave-uniform
-// index the compiler fetches x_j with scalar loads (one fetch per wave, no VGPRs).
+// index the compiler fetches x_j with scalar loads (one fetch per wave, no VGPRs).  Only
+// used by kernels during which the positions do not change.
 typedef const __attribute__((address_space(4))) double cdouble;
 
-__device__ __forceinline__ void pair32(const double qx, const double qy, const double px, const double py, float &tx,
-                                       float &ty) {
-    const float dx = (float)(qx - px);
-    const float dy = (float)(qy - py);
-    const float r2 = fmaf(dy, dy, fmaf(dx, dx, kTiny));
-    const float inv = __builtin_amdgcn_rcpf(r2);
-    tx = fmaf(-dx, inv, tx);
-    ty = fmaf(-dy, inv, ty);
+// 1/|d|^2 for a pair with FP64 difference (dx, dy).  REFINE: |d|^2 in FP64, the FP32
+// reciprocal as seed and one FP64 Newton step (relative error ~1e-14).  Otherwise |d|^2 and
+// the reciprocal in FP32 (relative error ~1e-7).  kTiny keeps the self pair at 0 * finite.
+template <bool REFINE>
+__device__ __forceinline__ double inv_r2(const double dx, const double dy) {
+    if constexpr (REFINE) {
+        const double r2 = fma(dy, dy, fma(dx, dx, (double)kTiny));
+        const double s = (double)__builtin_amdgcn_rcpf((float)r2);
+        return fma(s, fma(-r2, s, 1.0), s);
+    } else {
+        const float fx = (float)dx, fy = (float)dy;
+        const float r2 = fmaf(fy, fy, fmaf(fx, fx, kTiny));
+        return (double)__builtin_amdgcn_rcpf(r2);
+    }
 }
 
-// Dense FP32 repulsion + neighbour correction.  grid = (G + ncorr, rows).
-// Workgroups x < G: batch slots [y*IPW, y*IPW+IPW) against j chunk x (JS j-subgroups).
+// One repulsion pair of the FP32/mixed paths: -d/|d|^2 accumulated in FP64.
+template <bool REFINE>
+__device__ __forceinline__ void pair32(const double qx, const double qy, const double px, const double py, double &ax,
+                                       double &ay) {
+    const double dx = qx - px, dy = qy - py;
+    const double inv = inv_r2<REFINE>(dx, dy);
+    ax = fma(-dx, inv, ax);
+    ay = fma(-dy, inv, ay);
+}
+
+// Neighbour correction of the FP32/mixed paths for one (i, j) pair: removes the product the
+// dense loop accumulated for it (dx * inv with the same operands) and adds the attraction.
+template <bool REFINE>
+__device__ __forceinline__ void correct32(int law, double W, double dx, double dy, double &cx, double &cy) {
+    const double inv = inv_r2<REFINE>(dx, dy);
+    cx += dx * inv;
+    cy += dy * inv;
+    attract64(law, W, dx, dy, cx, cy);
+}
+
+// x_i += (F/|F|) * STEP and |F|^2, algorithms.cpp:775-779 / Coordinate.h:57-65.
+__device__ __forceinline__ void move_one(double2 *pos, int i, double fx, double fy, double step, double *eterm) {
+    const double mag = sqrt(fx * fx + fy * fy);
+    double2 p = pos[i];
+    p.x = p.x + (fx / mag) * step;
+    p.y = p.y + (fy / mag) * step;
+    pos[i] = p;
+    eterm[i] = fx * fx + fy * fy;
+}
+
+// ---------------------------------------------------------------------------------------
+// Persistent per-iteration kernel (FP32 mode)
+constexpr int NSTRIPE = 32;   // counters are striped over 32 cache lines
+constexpr int CSTRIDE = 32;   // uint32 between stripes (128 B)
+constexpr int MAX_SLOTS = 64; // batch slots per reducer workgroup
+
+struct IterArgs {
+    double2 *pos;
+    int n, B, nb, NW, NR, chunk, Bmax;
+    double step;
+    double2 *part;            // [2][Bmax][NW]
+    const int *erp, *eci;
+    const float *ew;
+    double W;
+    int law;
+    double *eterm;
+    unsigned *ctr;            // A stripes, then B stripes, then C stripes
+    int *err;
+    long long ticks;          // wait deadline in wall-clock ticks
+};
+
+__device__ __forceinline__ unsigned stripe_members(int total, int k) {
+    return (unsigned)(total / NSTRIPE + (k < total % NSTRIPE ? 1 : 0));
+}
+
+// Waits until every stripe k of ctr holds stripe_members(total, k) * rounds (acquire).
+// Returns false when the deadline passed or another workgroup flagged an error.
+__device__ bool wait_rounds(const unsigned *ctr, int total, unsigned rounds, int *err, long long ticks, int *sflag) {
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const unsigned target = lane < NSTRIPE ? stripe_members(total, lane) * rounds : 0u;
+        const long long t0 = wall_clock64();
+        int bad = 0;
+        for (;;) {
+            unsigned v = 0xffffffffu;
+            if (lane < NSTRIPE) v = __hip_atomic_load(ctr + lane * CSTRIDE, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (__all(v >= target)) break;
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                bad = 1;
+                break;
+            }
+            if (wall_clock64() - t0 > ticks) {
+                if (lane == 0) atomicOr(err, 1);
+                bad = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (lane == 0) *sflag = bad;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *sflag == 0;
+}
+
+// Publishes this workgroup's writes and bumps its stripe of ctr (release).
+__device__ void signal_done(unsigned *ctr, int who) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(ctr + (who % NSTRIPE) * CSTRIDE, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void dense_lds(const double2 *lds, int u0, int u1, double px, double py, double &ax,
+                                          double &ay) {
+    int u = u0;
+    for (; u + UNROLL <= u1; u += UNROLL) {
+#pragma unroll
+        for (int k = 0; k < UNROLL; ++k) {
+            const double2 q = lds[u + k];
+            pair32<true>(q.x, q.y, px, py, ax, ay);
+        }
+    }
+    for (; u < u1; ++u) {
+        const double2 q = lds[u];
+        pair32<true>(q.x, q.y, px, py, ax, ay);
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_iter32(IterArgs a) {
+    extern __shared__ double2 lds[];   // worker: the chunk's positions
+    __shared__ double2 red[WG / 64];
+    __shared__ double2 corr[MAX_SLOTS];
+    __shared__ int sflag;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned *ctrA = a.ctr, *ctrB = a.ctr + NSTRIPE * CSTRIDE, *ctrC = a.ctr + 2 * NSTRIPE * CSTRIDE;
+    const int w = blockIdx.x;
+    if (w < a.NW) {
+        // ---- worker: chunk [j0, j1) resident in LDS --------------------------------------
+        const int j0 = w * a.chunk, j1 = min(a.n, j0 + a.chunk);
+        const int cnt = max(0, j1 - j0);
+        for (int k = tid; k < cnt; k += WG) lds[k] = a.pos[j0 + k];
+        __syncthreads();
+        for (int b = 0; b < a.nb; ++b) {
+            const int b0 = b * a.B, bc = min(a.B, a.n - b0);
+            const int buf = b & 1;
+            // the reducers of step b-2 have read this partial buffer
+            if (b >= 2 && !wait_rounds(ctrB, a.NR, b - 1, a.err, a.ticks, &sflag)) return;
+            // part of the chunk moved by the previous batch: processed last, after its moves
+            int ov0 = j1, ov1 = j1;
+            if (b >= 1) {
+                ov0 = max(j0, b0 - a.B);
+                ov1 = min(j1, b0);
+                if (ov0 >= ov1) ov0 = ov1 = j1;
+            }
+            const int rounds = (bc + WG - 1) / WG;
+            for (int r = 0; r < rounds; ++r) {
+                const int slot = r * WG + tid;
+                const bool act = slot < bc;
+                const double2 pi = a.pos[b0 + (act ? slot : 0)];
+                double ax = 0.0, ay = 0.0;
+                dense_lds(lds, 0, ov0 - j0, pi.x, pi.y, ax, ay);
+                dense_lds(lds, ov1 - j0, cnt, pi.x, pi.y, ax, ay);
+                if (ov0 < ov1) {
+                    if (r == 0) {
+                        if (!wait_rounds(ctrB, a.NR, b, a.err, a.ticks, &sflag)) return;
+                        for (int k = ov0 + tid; k < ov1; k += WG) lds[k - j0] = a.pos[k];
+                        __syncthreads();
+                    }
+                    dense_lds(lds, ov0 - j0, ov1 - j0, pi.x, pi.y, ax, ay);
+                }
+                if (act) a.part[((size_t)buf * a.Bmax + slot) * a.NW + w] = make_double2(ax, ay);
+            }
+            signal_done(ctrA, w);
+        }
+        return;
+    }
+    // ---- reducer r: batch slots r, r+NR, ... -------------------------------------------------
+    const int r = w - a.NW;
+    for (int b = 0; b < a.nb; ++b) {
+        const int b0 = b * a.B, bc = min(a.B, a.n - b0);
+        const int buf = b & 1;
+        // every reducer has moved its vertices of batch b-1
+        if (b >= 1 && !wait_rounds(ctrB, a.NR, b, a.err, a.ticks, &sflag)) return;
+        const int myn = bc > r ? (bc - r + a.NR - 1) / a.NR : 0;
+        for (int k = wave; k < myn; k += WG / 64) {  // corrections, one wave per vertex
+            const int i = b0 + r + k * a.NR;
+            const double2 pi = a.pos[i];
+            double cx = 0.0, cy = 0.0;
+            const int kend = a.erp[i + 1];
+            for (int e = a.erp[i] + lane; e < kend; e += 64) {
+                const double2 pj = a.pos[a.eci[e]];
+                correct32<true>(a.law, a.ew ? a.W * (double)a.ew[e] : a.W, pj.x - pi.x, pj.y - pi.y, cx, cy);
+            }
+            cx = wave_sum(cx);
+            cy = wave_sum(cy);
+            if (lane == 0) corr[k] = make_double2(cx, cy);
+        }
+        signal_done(ctrC, r);
+        // all partials of batch b, and every correction of batch b (no vertex of the batch
+        // moves before all of them have read the batch's start positions)
+        if (!wait_rounds(ctrA, a.NW, b + 1, a.err, a.ticks, &sflag)) return;
+        if (!wait_rounds(ctrC, a.NR, b + 1, a.err, a.ticks, &sflag)) return;
+        for (int k = 0; k < myn; ++k) {
+            const int s = r + k * a.NR;
+            const double2 *row = a.part + ((size_t)buf * a.Bmax + s) * a.NW;
+            double sx = 0.0, sy = 0.0;
+            for (int p = tid; p < a.NW; p += WG) {
+                const double2 v = row[p];
+                sx += v.x;
+                sy += v.y;
+            }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            if (lane == 0) red[wave] = make_double2(sx, sy);
+            __syncthreads();
+            if (tid == 0) {
+                const double fx = ((red[0].x + red[1].x) + (red[2].x + red[3].x)) + corr[k].x;
+                const double fy = ((red[0].y + red[1].y) + (red[2].y + red[3].y)) + corr[k].y;
+                move_one(a.pos, b0 + s, fx, fy, a.step, a.eterm);
+            }
+            __syncthreads();
+        }
+        signal_done(ctrB, r);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-step kernels: the FP32 fallback and the FP64 reference-order path
+
+// Dense FP32 repulsion + neighbour correction (one launch per batch step).
+// grid = (G + ncorr, rows), 256 threads.  Workgroups x < G: batch slots [y*IPW, y*IPW+IPW)
+// against j chunk x, split over JS = 256/IPW j-subgroups added in LDS in subgroup order.
 // Workgroups x >= G (row 0 only): one wave per batch slot, neighbour correction.
 // part[slot * P + x] receives the chunk partial; part[slot * P + P-1] the correction.
-template <int JS>
-__global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos, int n, int b0, int bc,
-                                                int chunk, int G, double2 *__restrict__ part, int P,
+template <int JS, bool REFINE>
+__global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos, int n, int b0, int bc, int chunk,
+                                                int G, double2 *__restrict__ part, int P,
                                                 const int *__restrict__ erp, const int *__restrict__ eci,
                                                 const float *__restrict__ ew, double W, int law) {
+    constexpr int IPW = WG / JS;
     __shared__ double2 red[WG];
     const int tid = threadIdx.x;
     if ((int)blockIdx.x >= G) {
@@ -97,23 +324,14 @@ __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos,
         double cx = 0.0, cy = 0.0;
         const int kend = erp[i + 1];
         for (int k = erp[i] + lane; k < kend; k += 64) {
-            const int j = eci[k];
-            const double2 pj = pos[j];
-            const double dx = pj.x - pi.x, dy = pj.y - pi.y;
-            // the dense loop added fma(-dx32, inv32, acc): remove exactly -dx32*inv32
-            const float dx32 = (float)dx, dy32 = (float)dy;
-            const float r2 = fmaf(dy32, dy32, fmaf(dx32, dx32, kTiny));
-            const float inv = __builtin_amdgcn_rcpf(r2);
-            cx += (double)dx32 * (double)inv;
-            cy += (double)dy32 * (double)inv;
-            attract64(law, ew ? W * (double)ew[k] : W, dx, dy, cx, cy);
+            const double2 pj = pos[eci[k]];
+            correct32<REFINE>(law, ew ? W * (double)ew[k] : W, pj.x - pi.x, pj.y - pi.y, cx, cy);
         }
         cx = wave_sum(cx);
         cy = wave_sum(cy);
         if (lane == 0) part[(size_t)slot * P + (P - 1)] = make_double2(cx, cy);
         return;
     }
-    constexpr int IPW = WG / JS;
     const int li = tid % IPW;
     const int js = __builtin_amdgcn_readfirstlane(tid / IPW);  // IPW is a multiple of 64
     cdouble *cpos = (cdouble *)pos;
@@ -126,18 +344,13 @@ __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos,
     const int j0 = min(je, jb + js * sub);
     const int j1 = min(je, j0 + sub);
     double ax = 0.0, ay = 0.0;
-    for (int t0 = j0; t0 < j1; t0 += 64) {
-        const int t1 = min(j1, t0 + 64);
-        float tx = 0.f, ty = 0.f;
-        int u = t0;
-        for (; u + 8 <= t1; u += 8) {
+    int u = j0;
+    for (; u + UNROLL <= j1; u += UNROLL) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) pair32(cpos[2 * (u + k)], cpos[2 * (u + k) + 1], px, py, tx, ty);
-        }
-        for (; u < t1; ++u) pair32(cpos[2 * u], cpos[2 * u + 1], px, py, tx, ty);
-        ax += (double)tx;
-        ay += (double)ty;
+        for (int k = 0; k < UNROLL; ++k)
+            pair32<REFINE>(cpos[2 * (u + k)], cpos[2 * (u + k) + 1], px, py, ax, ay);
     }
+    for (; u < j1; ++u) pair32<REFINE>(cpos[2 * u], cpos[2 * u + 1], px, py, ax, ay);
     if constexpr (JS > 1) {
         red[tid] = make_double2(ax, ay);
         __syncthreads();
@@ -192,18 +405,8 @@ __global__ __launch_bounds__(WG) void k_pairs64(const double2 *__restrict__ pos,
     part[(size_t)slot * P + blockIdx.x] = make_double2(fx, fy);
 }
 
-// x_i += (F/|F|) * STEP and |F|^2, algorithms.cpp:775-779 / Coordinate.h:57-65.
-__device__ __forceinline__ void move_one(double2 *pos, int i, double fx, double fy, double step, double *eterm) {
-    const double mag = sqrt(fx * fx + fy * fy);
-    double2 p = pos[i];
-    p.x = p.x + (fx / mag) * step;
-    p.y = p.y + (fy / mag) * step;
-    pos[i] = p;
-    eterm[i] = fx * fx + fy * fy;
-}
-
-// FP32 mode: one workgroup per batch vertex sums its P partials (FP64, fixed order:
-// strided per thread, then a fixed tree), so every partial row is read by 256 lanes at once.
+// FP32 fallback: one workgroup per batch vertex sums its P partials (FP64, fixed order:
+// strided per thread, then a fixed tree).
 __global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part, int P, double2 *__restrict__ pos,
                                                int b0, int bc, double step, double *__restrict__ eterm) {
     __shared__ double2 red[WG / 64];
@@ -240,6 +443,41 @@ __global__ __launch_bounds__(WG) void k_move64(const double2 *__restrict__ part,
         sy += row[p].y;
     }
     move_one(pos, b0 + slot, sx, sy, step, eterm);
+}
+
+// Geometry of the persistent launch for a graph (0 workers = not possible).
+struct IterPlan {
+    int NW = 0, NR = 0, chunk = 0;
+    size_t lds = 0;
+};
+
+IterPlan plan_iter(int n, int bmax) {
+    IterPlan pl;
+    int dev = 0, cus = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0 ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop)
+        return pl;
+    pl.NR = std::min(256, std::max(32, (bmax + 7) / 8));
+    if ((bmax + pl.NR - 1) / pl.NR > MAX_SLOTS) return IterPlan();
+    for (int per_cu = 4; per_cu >= 1; --per_cu) {
+        int nw = per_cu * cus - pl.NR;
+        if (nw < 1) continue;
+        nw = std::min(nw, std::max(1, (n + 15) / 16));  // at least 16 vertices per chunk
+        const int chunk = (n + nw - 1) / nw;
+        const size_t lds = sizeof(double2) * (size_t)chunk;
+        if (lds > 64 * 1024) continue;
+        int fit = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_iter32, WG, lds) != hipSuccess) continue;
+        const int workers = (n + chunk - 1) / chunk;
+        if ((long)fit * cus >= (long)(workers + pl.NR)) {
+            pl.NW = workers;
+            pl.chunk = chunk;
+            pl.lds = lds;
+            return pl;
+        }
+    }
+    return IterPlan();
 }
 
 }  // namespace
@@ -290,30 +528,64 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     const float *ew = g->has_w ? g->eff_w.as<float>() : nullptr;
 
     // launch geometry
-    int JS = 1, rows = 1, G = 1, chunk = n, P = 1, ncorr = 0;
+    IterPlan pl;
+    int JS = 1, G = 1, chunk = n, P = 1;
+    size_t part_elems = 0;
     if (fp64) {
         G = (n + kBlockY - 1) / kBlockY;
         P = G;
-        rows = (bmax + WG - 1) / WG;
+        part_elems = (size_t)bmax * P;
     } else {
-        JS = bmax <= 64 ? 4 : (bmax <= 128 ? 2 : 1);
+        // per-step geometry (also the fallback when the persistent launch is refused)
+        // 64 batch slots x 4 j-subgroups per workgroup: ~1024 workgroups (4 per CU) with
+        // about 256 chunk partials per batch vertex for the move kernel to add up
+        JS = 4;
         const int ipw = WG / JS;
-        rows = (bmax + ipw - 1) / ipw;
-        const int target = (2048 + rows - 1) / rows;  // ~2048 workgroups per step
-        const int quant = 8 * JS;
+        const int rows = (bmax + ipw - 1) / ipw;
+        const int target = std::max(1, 1024 / rows);
+        const int quant = UNROLL * JS;
         chunk = (n + target - 1) / target;
         chunk = ((chunk + quant - 1) / quant) * quant;
-        if (chunk < 4 * quant) chunk = 4 * quant;
         G = (n + chunk - 1) / chunk;
         P = G + 1;
-        ncorr = (bmax + 3) / 4;
+        part_elems = (size_t)bmax * P;
+        if (getenv("BL_PERSISTENT") && p->precision == BL_PREC_MIXED) pl = plan_iter(n, bmax);
+        if (pl.NW > 0) part_elems = std::max(part_elems, 2 * (size_t)bmax * pl.NW);
     }
-    if (g->part.ensure(sizeof(double2) * (size_t)bmax * P)) return BL_E_NOMEM;
+    if (g->part.ensure(sizeof(double2) * part_elems)) return BL_E_NOMEM;
     if (g->eterm.ensure(sizeof(double) * (size_t)n)) return BL_E_NOMEM;
     if (g->energy.ensure(sizeof(double) * 4)) return BL_E_NOMEM;
+    const size_t ctr_bytes = sizeof(unsigned) * 3 * NSTRIPE * CSTRIDE + 256;
+    if (pl.NW > 0 && g->scratch.ensure(std::max(ctr_bytes, g->scratch.bytes))) return BL_E_NOMEM;
     double2 *part = g->part.as<double2>();
     double *eterm = g->eterm.as<double>();
     double *energy = g->energy.as<double>();
+
+    IterArgs ia;
+    if (pl.NW > 0) {
+        int dev = 0, khz = 0;
+        BL_HIP(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
+            khz = 100000;
+        ia.pos = pos;
+        ia.n = n;
+        ia.B = B;
+        ia.nb = nb;
+        ia.NW = pl.NW;
+        ia.NR = pl.NR;
+        ia.chunk = pl.chunk;
+        ia.Bmax = bmax;
+        ia.part = part;
+        ia.erp = erp;
+        ia.eci = eci;
+        ia.ew = ew;
+        ia.W = p->weight;
+        ia.law = law;
+        ia.eterm = eterm;
+        ia.ctr = g->scratch.as<unsigned>();
+        ia.err = reinterpret_cast<int *>(ia.ctr + 3 * NSTRIPE * CSTRIDE);
+        ia.ticks = (long long)khz * 1000LL * 2;  // 2 s per wait
+    }
 
     Prof prof;
     prof.on = p->profile != 0;
@@ -328,38 +600,54 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
             break;
         }
         E0 = E;
-        for (int b = 0; b < nb; ++b) {
-            const int b0 = b * B;
-            const int bc = (n - b0) < B ? (n - b0) : B;
-            if (fp64) {
-                prof.begin(0);
-                hipLaunchKernelGGL(k_pairs64, dim3(G, (bc + WG - 1) / WG), dim3(WG), 0, s, pos, n, b0, bc, part, P,
-                                   erp, eci, ew, p->weight, law);
-                prof.end();
-                prof.begin(1);
-                hipLaunchKernelGGL(k_move64, dim3((bc + WG - 1) / WG), dim3(WG), 0, s, part, P, pos, b0, bc, STEP,
-                                   eterm);
-                prof.end();
-            } else {
-                const int ipw = WG / JS;
-                const dim3 grid(G + (bc + 3) / 4, (bc + ipw - 1) / ipw);
-                prof.begin(0);
-                if (JS == 4)
-                    hipLaunchKernelGGL(k_pairs32<4>, grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part, P, erp,
-                                       eci, ew, p->weight, law);
-                else if (JS == 2)
-                    hipLaunchKernelGGL(k_pairs32<2>, grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part, P, erp,
-                                       eci, ew, p->weight, law);
-                else
-                    hipLaunchKernelGGL(k_pairs32<1>, grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part, P, erp,
-                                       eci, ew, p->weight, law);
-                prof.end();
-                prof.begin(1);
-                hipLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, part, P, pos, b0, bc, STEP, eterm);
-                prof.end();
+        bool done = false;
+        if (pl.NW > 0) {
+            BL_HIP(hipMemsetAsync(ia.ctr, 0, ctr_bytes, s));
+            ia.step = STEP;
+            void *args[] = {&ia};
+            prof.begin(0);
+            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_iter32),
+                                                            dim3(pl.NW + pl.NR), dim3(WG), args, (unsigned)pl.lds, s);
+            prof.end();
+            if (e == hipSuccess) {
+                done = true;
+            } else {  // refused (nothing ran): per-step launches from here on
+                (void)hipGetLastError();
+                fprintf(stderr, "batchlayout: persistent launch refused (%s); using per-step launches\n",
+                        hipGetErrorString(e));
+                pl.NW = 0;
             }
         }
-        (void)ncorr;
+        if (!done) {
+            for (int b = 0; b < nb; ++b) {
+                const int b0 = b * B;
+                const int bc = (n - b0) < B ? (n - b0) : B;
+                if (fp64) {
+                    prof.begin(0);
+                    hipLaunchKernelGGL(k_pairs64, dim3(G, (bc + WG - 1) / WG), dim3(WG), 0, s, pos, n, b0, bc, part,
+                                       P, erp, eci, ew, p->weight, law);
+                    prof.end();
+                    prof.begin(1);
+                    hipLaunchKernelGGL(k_move64, dim3((bc + WG - 1) / WG), dim3(WG), 0, s, part, P, pos, b0, bc,
+                                       STEP, eterm);
+                    prof.end();
+                } else {
+                    const int ipw = WG / JS;
+                    const dim3 grid(G + (bc + 3) / 4, (bc + ipw - 1) / ipw);
+                    prof.begin(0);
+                    if (p->precision == BL_PREC_FP32)
+                        hipLaunchKernelGGL((k_pairs32<4, false>), grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G,
+                                           part, P, erp, eci, ew, p->weight, law);
+                    else
+                        hipLaunchKernelGGL((k_pairs32<4, true>), grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G,
+                                           part, P, erp, eci, ew, p->weight, law);
+                    prof.end();
+                    prof.begin(1);
+                    hipLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, part, P, pos, b0, bc, STEP, eterm);
+                    prof.end();
+                }
+            }
+        }
         prof.begin(2);
         int rc = launch_energy(eterm, n, energy, p->precision, s);
         prof.end();
@@ -376,7 +664,13 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     if (!converged && loop > 0) {
         BL_HIP(hipMemcpyAsync(&E, energy, sizeof(double), hipMemcpyDeviceToHost, s));
     }
+    int err = 0;
+    if (pl.NW > 0 && loop > 0) BL_HIP(hipMemcpyAsync(&err, ia.err, sizeof(int), hipMemcpyDeviceToHost, s));
     BL_HIP(hipStreamSynchronize(s));
+    if (err) {
+        fprintf(stderr, "batchlayout: a wait of the persistent minibatch kernel passed its deadline\n");
+        return BL_E_DEVICE;
+    }
     prof.report(st);
     if (st) {
         st->energy = loop > 0 ? E : (law == LAW_FR ? 0.0 : 1.79769313486231570815e+308);

@@ -8,13 +8,15 @@ uniform in [0,1)^2 (seed 0); algo 2 (cache-blocking minibatch), -batch 256, 100 
 
 A "step" is one layout iteration (every vertex updated once, ceil(n/256) batch steps).
 `value` = vertex-updates/s = n * steps / seconds of the timed region, positions resident in
-HBM before the region starts.  The energy of the FP32 run is checked against the
+HBM before the region starts.  The energy of the default run is checked against the
 reference-order FP64 path run on the same GPU from the same start (`energy` field).
 
 Also reported:
-  roofline      the dominant kernel (all-pairs repulsion, k_pairs32): 10 FP32 flops per
-                pair x pairs per launch / mean launch time from HIP events on the launch
-                stream, against the MI355X FP32 peak (157.3 TFLOP/s)
+  roofline      the dominant kernel (all-pairs repulsion, k_pairs32): 10 flops per pair x
+                pairs per launch / mean launch time from HIP events on the launch stream,
+                against the MI355X FP64 vector peak (78.6 TFLOP/s; the default pair arithmetic
+                is FP64 apart from the FP32 reciprocal seed)
+  fp32_mode     the same workload with FP32 |d|^2 and reciprocal (against 157.3 TFLOP/s)
   cpu_baseline  the reference built from its own sources (oracle/_ref/ref_fast) or, when
                 that binary is absent, the C oracle, timed on this host's cores for one
                 iteration of the same workload
@@ -34,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "vertex-updates/s at energy within 1% of FP64 ref; learned vs plain BH, 1 GPU"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 FLOPS_PER_PAIR = 10
 
 
@@ -107,54 +110,68 @@ def main():
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
     K, W, B = args.steps, args.warmup, args.batch
-    with api.Graph(rp, ci) as g:
-        if W > 0:
-            d_xy.copy_(d_init)
-            g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=W, batch=B)
-        # timed region
+
+    def timed(g, precision, profile=0):
         d_xy.copy_(d_init)
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
-        st = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B)
+        st = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B, precision=precision,
+                          profile=profile)
         ev1.record(stream)
         torch.cuda.synchronize()
-        secs = ev0.elapsed_time(ev1) / 1e3
-        e32 = st.energy
-        log(f"timed: {K} iterations in {secs:.3f} s, E={e32:.6e}")
+        return ev0.elapsed_time(ev1) / 1e3, st
+
+    with api.Graph(rp, ci) as g:
+        for prec in (api.PREC_MIXED, api.PREC_FP32):
+            if W > 0:
+                d_xy.copy_(d_init)
+                g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=W, batch=B, precision=prec)
+        # timed region (default precision)
+        secs, st = timed(g, api.PREC_MIXED)
+        log(f"timed: {K} iterations in {secs:.3f} s, E={st.energy:.6e}")
         # profiled pass: per-launch HIP events on the same stream
-        d_xy.copy_(d_init)
-        torch.cuda.synchronize()
-        sp_prof = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B, profile=1)
+        _, sp_prof = timed(g, api.PREC_MIXED, profile=1)
         pair_ms = sp_prof.kernel_ms[0] / max(1, sp_prof.kernel_launches[0])
         move_ms = sp_prof.kernel_ms[1] / max(1, sp_prof.kernel_launches[1])
         log(f"profiled: pair kernel {pair_ms*1e3:.2f} us/launch x {sp_prof.kernel_launches[0]}, "
             f"move {move_ms*1e3:.2f} us/launch")
-        energy = dict(fp32=e32)
+        # the FP32 mode, same workload
+        secs32, st32 = timed(g, api.PREC_FP32)
+        _, sp32 = timed(g, api.PREC_FP32, profile=1)
+        pair32_ms = sp32.kernel_ms[0] / max(1, sp32.kernel_launches[0])
+        log(f"fp32 mode: {K} iterations in {secs32:.3f} s, E={st32.energy:.6e}, pair kernel {pair32_ms*1e3:.2f} us")
+        energy = dict(mixed=st.energy, fp32=st32.energy)
         if not args.no_fp64_check:
             d_xy.copy_(d_init)
             torch.cuda.synchronize()
             t1 = time.time()
             s64 = g.run_device(d_xy.data_ptr(), stream=sp, algo=2, iters=K, batch=B, precision=api.PREC_FP64)
             torch.cuda.synchronize()
-            rel = abs(e32 - s64.energy) / s64.energy
-            energy.update(fp64_gpu=s64.energy, rel_diff=rel, within_1pct=bool(rel <= 0.01),
-                          fp64_seconds=time.time() - t1)
-            log(f"fp64 check: E64={s64.energy:.6e} rel diff {rel:.2e}")
+            rel = abs(st.energy - s64.energy) / s64.energy
+            rel32 = abs(st32.energy - s64.energy) / s64.energy
+            energy.update(fp64_reference_order=s64.energy, rel_diff=rel, within_1pct=bool(rel <= 0.01),
+                          fp32_rel_diff=rel32, fp64_seconds=time.time() - t1)
+            log(f"fp64 check: E64={s64.energy:.6e} rel diff mixed {rel:.2e} fp32 {rel32:.2e}")
 
     nb = (n + B - 1) // B
     flops_per_launch = FLOPS_PER_PAIR * n * (n - 1) / nb
     achieved = flops_per_launch / (pair_ms / 1e3) / 1e12
-    roofline = dict(bound="valu", kernel="k_pairs32", achieved=achieved, peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP32_TFLOPS, traffic=None, mean_launch_us=pair_ms * 1e3,
-                    launches=sp_prof.kernel_launches[0], flops_per_launch=flops_per_launch)
+    roofline = dict(bound="valu-fp64", kernel="k_pairs32<4,true>", achieved=achieved, peak=PEAK_FP64_TFLOPS,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP64_TFLOPS, traffic=None, mean_launch_us=pair_ms * 1e3,
+                    launches=sp_prof.kernel_launches[0], flops_per_launch=flops_per_launch,
+                    flops_per_pair=FLOPS_PER_PAIR)
+    achieved32 = flops_per_launch / (pair32_ms / 1e3) / 1e12
     out = dict(metric=METRIC, value=n * K / secs, unit="vertex-updates/s", n_gpus=1, steps=K, warmup=W,
-               ms_per_step=secs * 1e3 / K, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="fp32",
+               ms_per_step=secs * 1e3 / K, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
                data="synthetic",
                config=dict(workload="rgg250k-algo2", graph="random geometric, 250,000 points, seed 1, mean degree 8, "
                            "largest component", n=n, nnz=int(len(ci)), batch=B, iterations=K, init="uniform seed 0",
-                           parallelism="none (1 GPU)"),
-               roofline=roofline, energy=energy)
+                           precision="mixed: FP64 differences and sums, FP32-seeded reciprocal with one FP64 "
+                                     "Newton step", parallelism="none (1 GPU)"),
+               roofline=roofline, energy=energy,
+               fp32_mode=dict(value=n * K / secs32, ms_per_step=secs32 * 1e3 / K, pair_kernel_us=pair32_ms * 1e3,
+                              achieved_tflops=achieved32, frac_of_fp32_peak=achieved32 / PEAK_FP32_TFLOPS))
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         log(f"cpu baseline on {threads} threads ...")

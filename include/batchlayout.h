@@ -45,8 +45,10 @@ extern "C" {
 #define BL_ALGO_FA 8         /* FAcacheBlockingminiBatchForceDirectedAlgorithm, :947-1018 */
 
 /* ---- precision of the pair arithmetic -------------------------------------------- */
-#define BL_PREC_FP32 0       /* fast path: FP32 pair math, FP64 coordinate differences, FP64 tile sums */
+#define BL_PREC_MIXED 0      /* default: FP64 differences and sums; 1/|d|^2 seeded by the FP32
+                                reciprocal and refined by one FP64 Newton step (FP64-grade terms) */
 #define BL_PREC_FP64 1       /* reference-grade path: the reference's FP64 operations in its order */
+#define BL_PREC_FP32 2       /* fastest: |d|^2 and 1/|d|^2 in FP32, FP64 differences and sums */
 
 typedef struct bl_graph bl_graph;
 
@@ -64,7 +66,7 @@ typedef struct {
     double weight;      /* -weight, the global attraction scale W */
     double step0;       /* STEP of the first iteration (1.0 as in the reference; a resumed
                            or teacher-forced run passes the STEP it continues from) */
-    int32_t precision;  /* BL_PREC_* */
+    int32_t precision;  /* BL_PREC_* (the Barnes-Hut path treats MIXED like FP32) */
     int32_t profile;    /* nonzero: time every launch with HIP events on the run's stream and
                            report per-kernel-class totals in bl_stats (adds a sync per iteration) */
 } bl_params;

@@ -114,7 +114,7 @@ def test_algo4_final_energy_600_iterations(bl):
     rp, ci = bl.mtx_read(golden_input("grid64x64.mtx"))
     with bl.Graph(rp, ci) as g:
         print("[case] grid64 algo 4 600 iterations", flush=True)
-        xy, st = g.run(init, algo=4, iters=600, batch=256, precision=bl.PREC_FP32)
+        xy, st = g.run(init, algo=4, iters=600, batch=256)
     rel = abs(st.energy - e_ref) / e_ref
     print(f"algo 4: E={st.energy:.6e} ref={e_ref:.6e} rel={rel:.3e} tol={tol:.3e} ({st.seconds*1e3:.1f} ms)")
     assert rel <= tol
