@@ -71,7 +71,7 @@ def lib():
                                  _f64p, _f64p, _f64p, _i32p]
         L.bl_bh_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, _f64p, _f64p, C.c_int32]
         L.bl_run_batched.argtypes = [C.c_int32, _i32p, _i64p, _i32p, _i32p, _f64p, C.POINTER(bl_params), _f64p,
-                                     C.POINTER(bl_stats)]
+                                     _f64p, C.POINTER(bl_stats)]
         L.bl_device_info.argtypes = [C.c_char_p, C.c_int32]
         _lib = L
     return _lib
@@ -217,7 +217,10 @@ def bh_forces(xy, theta, queries=None, precision=PREC_FP32):
 
 
 def run_batched(graphs, xys, **kw):
-    """Lays out a list of (rowptr, colids) graphs in one launch; returns (list of xy, energies, Stats)."""
+    """Lays out a list of (rowptr, colids) graphs in one launch.
+
+    Returns (list of xy, energies, finish_ms, Stats); finish_ms[k] is graph k's completion
+    time measured from the start of the launch."""
     count = len(graphs)
     voff = np.zeros(count + 1, np.int32)
     eoff = np.zeros(count + 1, np.int64)
@@ -231,12 +234,13 @@ def run_batched(graphs, xys, **kw):
         colids[eoff[k]:eoff[k + 1]] = ci
     xy = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64).reshape(-1, 2) for x in xys]))
     energies = np.zeros(count)
+    finish = np.zeros(count)
     p = params(**kw)
     st = bl_stats()
-    _chk(lib().bl_run_batched(count, voff, eoff, rowptr, colids, xy, C.byref(p), energies, C.byref(st)),
+    _chk(lib().bl_run_batched(count, voff, eoff, rowptr, colids, xy, C.byref(p), energies, finish, C.byref(st)),
          "bl_run_batched")
     out = [xy[voff[k]:voff[k + 1]].copy() for k in range(count)]
-    return out, energies, _stats(st)
+    return out, energies, finish, _stats(st)
 
 
 def device_info() -> str:

@@ -103,37 +103,11 @@ extern "C" int bl_graph_create(int32_t n, int64_t nnz, const int32_t *rowptr, co
     int rc = check_device_once();
     if (rc) return rc;
 
-    // Neighbours the merge pointer of algorithms.cpp:758-762 meets: the pointer starts at
-    // rowptr[i], matches when j == colids[ptr] and advances only while ptr < rowptr[i+1]-1.
-    // With j ascending that is the longest strictly increasing prefix of the row; an empty
-    // row keeps the pointer on the next row's first entry, which therefore acts as one
-    // neighbour (an empty last row would read past the array: no neighbour here).
     std::vector<int32_t> erp(n + 1, 0), eci;
     std::vector<float> ew;
     eci.reserve((size_t)nnz);
     int32_t maxdeg = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        const int64_t s = rowptr[i], e = rowptr[i + 1];
-        int32_t cnt = 0;
-        if (s < e) {
-            int32_t last = colids[s];
-            eci.push_back(last);
-            if (w_edge) ew.push_back(w_edge[s]);
-            ++cnt;
-            for (int64_t k = s + 1; k < e && colids[k] > last; ++k) {
-                last = colids[k];
-                eci.push_back(last);
-                if (w_edge) ew.push_back(w_edge[k]);
-                ++cnt;
-            }
-        } else if (s < nnz) {
-            eci.push_back(colids[s]);
-            if (w_edge) ew.push_back(w_edge[s]);
-            ++cnt;
-        }
-        erp[i + 1] = erp[i] + cnt;
-        if (cnt > maxdeg) maxdeg = cnt;
-    }
+    effective_neighbors(rowptr, colids, w_edge, 0, n, nnz, erp, eci, ew, &maxdeg);
 
     bl_graph *g = new (std::nothrow) bl_graph();
     if (!g) return BL_E_NOMEM;
@@ -308,14 +282,25 @@ extern "C" int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t n
 
 extern "C" int bl_run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,
                               const int32_t *colids, double *xy_inout, const bl_params *p, double *energies,
-                              bl_stats *st) {
+                              double *finish_ms, bl_stats *st) {
     if (count <= 0 || !voff || !eoff || !rowptr || !xy_inout || !p) return BL_E_ARG;
     int rc = validate_params(p);
     if (rc) return rc;
     if (p->algo != BL_ALGO_CACHE && p->algo != BL_ALGO_FA) return BL_E_ARG;
+    if (p->precision == BL_PREC_FP64 || p->converged) return BL_E_ARG;
+    if (voff[0] != 0) return BL_E_ARG;
+    for (int32_t k = 0; k < count; ++k) {  // per-graph CSR validation
+        const int32_t n = voff[k + 1] - voff[k];
+        if (n <= 0 || n > BL_BATCHED_MAX_N) return BL_E_ARG;
+        if (rowptr[voff[k]] != eoff[k] || rowptr[voff[k + 1]] != eoff[k + 1]) return BL_E_GRAPH;
+        for (int32_t i = voff[k]; i < voff[k + 1]; ++i)
+            if (rowptr[i + 1] < rowptr[i]) return BL_E_GRAPH;
+        for (int64_t e = eoff[k]; e < eoff[k + 1]; ++e)
+            if (colids[e] < 0 || colids[e] >= n) return BL_E_GRAPH;
+    }
     rc = check_device_once();
     if (rc) return rc;
-    return run_batched(count, voff, eoff, rowptr, colids, xy_inout, p, energies, st);
+    return run_batched(count, voff, eoff, rowptr, colids, xy_inout, p, energies, finish_ms, st);
 }
 
 extern "C" int bl_device_info(char *buf, int32_t len) {

@@ -1,9 +1,275 @@
-// Batched small graphs: placeholder until the persistent batched kernel lands.
+// Many small graphs in one persistent launch (bl_run_batched).
+//
+// Each graph follows the single-graph minibatch algorithm (algorithms.cpp:715-794 for algo 2,
+// :947-1018 for algo 8): ceil(n/B) batches per iteration, Jacobi inside a batch, Gauss-Seidel
+// across batches, x_i += STEP*F/|F|, STEP *= decay, with the pair arithmetic and neighbour
+// correction of pair_math.h.  Workgroups of 1024 threads pull graphs from an atomic work
+// queue (largest n^2 first) and run all of a graph's iterations alone: the graph's FP64
+// positions sit in LDS when they fit (read by every lane with the same address, i.e.
+// broadcast) and are read from global memory otherwise.  A graph never shares data with
+// another workgroup, so the only cross-workgroup traffic is the queue counter.
+//
+// Inside a batch step the IPW = next power of two >= min(B, n) batch slots are split over
+// JS = 1024 / IPW j-subgroups; the subgroup partials are added in subgroup order, then the
+// neighbour correction, so results do not depend on timing.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
 #include "bl_internal.h"
+#include "pair_math.h"
 
 namespace bl {
-int run_batched(int32_t, const int32_t *, const int64_t *, const int32_t *, const int32_t *, double *,
-                const bl_params *, double *, bl_stats *) {
-    return BL_E_ARG;
+namespace {
+
+constexpr int BT = 1024;       // threads per workgroup
+constexpr int MAXB = 1024;     // largest batch size of the batched path
+
+struct BatchedArgs {
+    const int *order;          // graph ids, largest first
+    int count;
+    const int *voff;           // vertex offsets [count+1]
+    const int *erp;            // effective-neighbour rowptr over all vertices (global entry ids)
+    const int *eci;            // effective neighbours (graph-local vertex ids)
+    double2 *pos;              // all positions
+    int B, iters, law, refine, lds_cap;
+    double step0, decay, W;
+    double *energies;
+    long long *finish;         // wall-clock tick at each graph's completion
+    long long *wg_start;       // wall-clock tick at each workgroup's start
+    int *queue;
+};
+
+template <bool REFINE>
+__device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *red, double2 *fb, double *eb) {
+    const int tid = threadIdx.x;
+    const int v0 = a.voff[g], n = a.voff[g + 1] - v0;
+    const bool in_lds = n <= a.lds_cap;
+    double2 *gp = in_lds ? P : a.pos + v0;
+    if (in_lds) {
+        for (int k = tid; k < n; k += BT) P[k] = a.pos[v0 + k];
+        __syncthreads();
+    }
+    const int B = a.B;
+    const int bmax = min(B, n);
+    int ipw = 64;
+    while (ipw < bmax && ipw < BT) ipw <<= 1;
+    const int JS = BT / ipw;
+    const int li = tid % ipw, js = tid / ipw;
+    const int sub = (n + JS - 1) / JS;
+    const int j0 = min(n, js * sub), j1 = min(n, j0 + sub);
+    const int nb = (n + B - 1) / B;
+    double step = a.step0, E = 0.0;
+    for (int it = 0; it < a.iters; ++it) {
+        E = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            const int b0 = b * B, bc = min(B, n - b0);
+            for (int r0 = 0; r0 < bc; r0 += ipw) {  // rounds of ipw slots
+                const int slot = r0 + li;
+                const bool act = slot < bc;
+                const double2 pi = gp[b0 + (act ? slot : 0)];
+                double ax = 0.0, ay = 0.0;
+                for (int u = j0; u < j1; ++u) {
+                    const double2 q = gp[u];
+                    pair_rep<REFINE>(q.x, q.y, pi.x, pi.y, ax, ay);
+                }
+                red[tid] = make_double2(ax, ay);
+                __syncthreads();
+                if (js == 0 && act) {
+                    for (int s = 1; s < JS; ++s) {
+                        ax += red[s * ipw + li].x;
+                        ay += red[s * ipw + li].y;
+                    }
+                    const int i = v0 + b0 + slot;
+                    double cx = 0.0, cy = 0.0;
+                    for (int e = a.erp[i]; e < a.erp[i + 1]; ++e) {
+                        const double2 pj = gp[a.eci[e]];
+                        correct_pair<REFINE>(a.law, a.W, pj.x - pi.x, pj.y - pi.y, cx, cy);
+                    }
+                    fb[slot] = make_double2(ax + cx, ay + cy);
+                }
+                __syncthreads();
+            }
+            // move the batch (every force of the batch is formed: Jacobi inside the batch)
+            double e2 = 0.0;
+            for (int slot = tid; slot < bc; slot += BT) {
+                const double2 f = fb[slot];
+                gp[b0 + slot] = moved(gp[b0 + slot], f.x, f.y, step);
+                e2 += f.x * f.x + f.y * f.y;
+            }
+            eb[tid] = e2;
+            __syncthreads();
+            for (int w = BT / 2; w > 0; w >>= 1) {
+                if (tid < w) eb[tid] += eb[tid + w];
+                __syncthreads();
+            }
+            E += eb[0];  // batches added in order
+            __syncthreads();
+        }
+        step = step * a.decay;
+    }
+    if (in_lds) {
+        for (int k = tid; k < n; k += BT) a.pos[v0 + k] = P[k];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        a.energies[g] = E;
+        a.finish[g] = wall_clock64();
+    }
 }
+
+__global__ __launch_bounds__(BT) void k_batched(BatchedArgs a) {
+    extern __shared__ double2 P[];  // positions of the current graph when it fits
+    __shared__ double2 red[BT];
+    __shared__ double2 fb[MAXB];
+    __shared__ double eb[BT];
+    __shared__ int slot;
+    const int tid = threadIdx.x;
+    if (tid == 0) a.wg_start[blockIdx.x] = wall_clock64();
+    for (;;) {
+        if (tid == 0) slot = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int q = slot;
+        __syncthreads();
+        if (q >= a.count) break;
+        const int g = a.order[q];
+        if (a.refine)
+            layout_graph<true>(a, g, P, red, fb, eb);
+        else
+            layout_graph<false>(a, g, P, red, fb, eb);
+    }
+}
+
+}  // namespace
+
+int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr, const int32_t *colids,
+                double *xy, const bl_params *p, double *energies, double *finish_ms, bl_stats *st) {
+    if (p->batch > MAXB) return BL_E_ARG;
+    const int N = voff[count];
+    // effective neighbours per graph, each graph's own entry range being its array end
+    std::vector<int32_t> erp(N + 1, 0), eci;
+    std::vector<float> unused;
+    eci.reserve((size_t)eoff[count]);
+    for (int32_t k = 0; k < count; ++k)
+        effective_neighbors(rowptr, colids, nullptr, voff[k], voff[k + 1], eoff[k + 1], erp, eci, unused, nullptr);
+    // largest graphs first
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const long long nx = voff[x + 1] - voff[x], ny = voff[y + 1] - voff[y];
+        return nx * nx > ny * ny;
+    });
+
+    int dev = 0, cus = 0, khz = 0, maxlds = 0;
+    BL_HIP(hipGetDevice(&dev));
+    BL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BL_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+    const int static_lds = (int)(sizeof(double2) * (BT + MAXB) + sizeof(double) * BT + 64);
+    int maxn = 0;
+    for (int k = 0; k < count; ++k) maxn = std::max(maxn, voff[k + 1] - voff[k]);
+    const int cap = std::max(0, std::min(maxn, (maxlds - static_lds) / (int)sizeof(double2)));
+    const size_t dyn = sizeof(double2) * (size_t)cap;
+    if (dyn > 0) BL_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    int per_cu = 0;
+    BL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched, BT, dyn));
+    const int grid = std::max(1, std::min(count, std::max(1, per_cu) * cus));
+
+    hipStream_t s = nullptr;
+    BL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct Buffers {
+        void *p = nullptr;
+        hipStream_t s = nullptr;
+        ~Buffers() {
+            if (s) (void)hipStreamSynchronize(s);
+            if (p) (void)hipFree(p);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } buf;
+    buf.s = s;
+    const size_t b_order = 4 * (size_t)count, b_voff = 4 * (size_t)(count + 1), b_erp = 4 * (size_t)(N + 1),
+                 b_eci = 4 * std::max<size_t>(1, eci.size()), b_pos = 16 * (size_t)N, b_en = 8 * (size_t)count,
+                 b_fin = 8 * (size_t)count, b_wg = 8 * (size_t)grid, b_q = 64;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = al(b_order) + al(b_voff) + al(b_erp) + al(b_eci) + al(b_pos) + al(b_en) + al(b_fin) +
+                         al(b_wg) + al(b_q);
+    BL_HIP(hipMalloc(&buf.p, total));
+    char *c = static_cast<char *>(buf.p);
+    int *d_order = reinterpret_cast<int *>(c); c += al(b_order);
+    int *d_voff = reinterpret_cast<int *>(c); c += al(b_voff);
+    int *d_erp = reinterpret_cast<int *>(c); c += al(b_erp);
+    int *d_eci = reinterpret_cast<int *>(c); c += al(b_eci);
+    double2 *d_pos = reinterpret_cast<double2 *>(c); c += al(b_pos);
+    double *d_en = reinterpret_cast<double *>(c); c += al(b_en);
+    long long *d_fin = reinterpret_cast<long long *>(c); c += al(b_fin);
+    long long *d_wg = reinterpret_cast<long long *>(c); c += al(b_wg);
+    int *d_q = reinterpret_cast<int *>(c);
+    BL_HIP(hipMemcpyAsync(d_order, order.data(), b_order, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(d_voff, voff, b_voff, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(d_erp, erp.data(), b_erp, hipMemcpyHostToDevice, s));
+    if (!eci.empty()) BL_HIP(hipMemcpyAsync(d_eci, eci.data(), 4 * eci.size(), hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(d_pos, xy, b_pos, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemsetAsync(d_q, 0, b_q, s));
+
+    BatchedArgs a;
+    a.order = d_order;
+    a.count = count;
+    a.voff = d_voff;
+    a.erp = d_erp;
+    a.eci = d_eci;
+    a.pos = d_pos;
+    a.B = p->batch;
+    a.iters = p->iters;
+    a.law = p->algo == BL_ALGO_FA ? LAW_FA : LAW_FR;
+    a.refine = p->precision == BL_PREC_MIXED ? 1 : 0;
+    a.lds_cap = cap;
+    a.step0 = p->step0;
+    a.decay = p->algo == BL_ALGO_FA ? 0.99 : 0.999;
+    a.W = p->weight;
+    a.energies = d_en;
+    a.finish = d_fin;
+    a.wg_start = d_wg;
+    a.queue = d_q;
+    hipEvent_t e0, e1;
+    BL_HIP(hipEventCreate(&e0));
+    BL_HIP(hipEventCreate(&e1));
+    BL_HIP(hipEventRecord(e0, s));
+    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(BT), dyn, s, a);
+    BL_HIP(hipGetLastError());
+    BL_HIP(hipEventRecord(e1, s));
+    std::vector<double> en(count);
+    std::vector<long long> fin(count), wg(grid);
+    BL_HIP(hipMemcpyAsync(xy, d_pos, b_pos, hipMemcpyDeviceToHost, s));
+    BL_HIP(hipMemcpyAsync(en.data(), d_en, b_en, hipMemcpyDeviceToHost, s));
+    BL_HIP(hipMemcpyAsync(fin.data(), d_fin, b_fin, hipMemcpyDeviceToHost, s));
+    BL_HIP(hipMemcpyAsync(wg.data(), d_wg, b_wg, hipMemcpyDeviceToHost, s));
+    BL_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (energies) std::copy(en.begin(), en.end(), energies);
+    if (finish_ms) {
+        const long long t0 = *std::min_element(wg.begin(), wg.end());
+        for (int k = 0; k < count; ++k) finish_ms[k] = (double)(fin[k] - t0) / (double)khz;
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->energy = std::accumulate(en.begin(), en.end(), 0.0);
+        st->iters_done = p->iters;
+        st->seconds = ms / 1e3;
+        double sstep = p->step0;
+        for (int it = 0; it < p->iters; ++it) sstep *= a.decay;
+        st->step_final = sstep;
+        st->kernel_ms[0] = ms;
+        st->kernel_launches[0] = 1;
+    }
+    return BL_OK;
+}
+
 }  // namespace bl

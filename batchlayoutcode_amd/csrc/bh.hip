@@ -654,6 +654,105 @@ __global__ __launch_bounds__(1024) void k_attract_move(const int *__restrict__ r
     }
 }
 
+// Same batch walk for the FP32/mixed modes with the attraction work of each batch split
+// evenly over the 1024 threads by CSR entry (a hub row of tens of thousands of entries is
+// shared by many threads instead of one).  Thread t takes the entries [E0 + t*c, E0 + t*c + c)
+// of the batch; a row lying inside one thread's range is summed by that thread; a row spread
+// over several threads is the sum, in thread order, of those threads' partial sums (the
+// range's last row of the first thread, then whole ranges, then the first row of the last
+// thread), so the result does not depend on timing.
+__global__ __launch_bounds__(1024) void k_attract_move_bal(const int *__restrict__ rowptr,
+                                                           const int *__restrict__ colids,
+                                                           const float *__restrict__ w_edge, double W,
+                                                           double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                           double2 *__restrict__ fbuf, int n, int B, double step,
+                                                           double *__restrict__ eterm, const int *__restrict__ status) {
+    constexpr int T = 1024;
+    __shared__ double2 head[T], tail[T];
+    __shared__ int hrow[T], trow[T];
+    if (*status) return;
+    const int t = threadIdx.x;
+    const int nb = (n + B - 1) / B;
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = b * B, b1 = min(n, b0 + B);
+        const int E0 = rowptr[b0], E1 = rowptr[b1];
+        const int total = E1 - E0;
+        const int c = max(1, (total + T - 1) / T);
+        const int s = E0 + t * c, e = min(E1, s + c);
+        hrow[t] = -1;
+        trow[t] = -1;
+        if (s < e) {
+            int lo = b0, hi = b1;  // row containing entry s: last r with rowptr[r] <= s
+            while (hi - lo > 1) {
+                const int m = (lo + hi) >> 1;
+                if (rowptr[m] <= s) lo = m; else hi = m;
+            }
+            int r = lo;
+            const int first = r;
+            int rend = rowptr[r + 1];
+            double2 pi = pos[r];
+            double ax = 0.0, ay = 0.0;
+            for (int k = s; k < e; ++k) {
+                while (k >= rend) {  // row r complete inside this range
+                    if (r == first) {
+                        hrow[t] = r;
+                        head[t] = make_double2(ax, ay);
+                    } else {
+                        fbuf[r] = make_double2(ax, ay);
+                    }
+                    ++r;
+                    rend = rowptr[r + 1];
+                    pi = pos[r];
+                    ax = ay = 0.0;
+                }
+                const double2 pj = pos[colids[k]];
+                const double dx = pj.x - pi.x, dy = pj.y - pi.y;
+                const double Wk = w_edge ? W * (double)w_edge[k] : W;
+                const double sc = Wk * sqrt(dx * dx + dy * dy);
+                ax = fma(dx, sc, ax);
+                ay = fma(dy, sc, ay);
+            }
+            if (r == first) {
+                hrow[t] = r;
+                head[t] = make_double2(ax, ay);
+            } else {
+                trow[t] = r;
+                tail[t] = make_double2(ax, ay);
+            }
+        }
+        __syncthreads();
+        for (int i = b0 + t; i < b1; i += T) {
+            const int rs = rowptr[i], re = rowptr[i + 1];
+            double2 f = make_double2(0.0, 0.0);
+            if (re > rs) {
+                const int ta = (rs - E0) / c, tb = (re - 1 - E0) / c;
+                if (ta == tb) {
+                    f = hrow[ta] == i ? head[ta] : (trow[ta] == i ? tail[ta] : fbuf[i]);
+                } else {
+                    f = trow[ta] == i ? tail[ta] : head[ta];
+                    for (int q = ta + 1; q <= tb; ++q) {
+                        f.x += head[q].x;
+                        f.y += head[q].y;
+                    }
+                }
+            }
+            const double2 rr = rep[i];
+            fbuf[i] = make_double2(f.x - rr.x, f.y - rr.y);
+        }
+        __syncthreads();
+        for (int i = b0 + t; i < b1; i += T) {
+            const double2 f = fbuf[i];
+            const double mag = sqrt(f.x * f.x + f.y * f.y);
+            double2 q = pos[i];
+            q.x = q.x + (f.x / mag) * step;
+            q.y = q.y + (f.y / mag) * step;
+            pos[i] = q;
+            eterm[i] = f.x * f.x + f.y * f.y;
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // host orchestration
 struct TreeBufs {
@@ -781,9 +880,16 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         prof.end();
         if (rc) return rc;
         prof.begin(1);
-        hipLaunchKernelGGL(k_attract_move, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
-                           g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
-                           g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
+        if (p->precision == BL_PREC_FP64)
+            hipLaunchKernelGGL(k_attract_move, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
+                               g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
+                               g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
+                               g->bh.misc.as<int>());
+        else
+            hipLaunchKernelGGL(k_attract_move_bal, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(),
+                               g->colids.as<int>(), g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos,
+                               g->bh.rep.as<double2>(), g->scratch.as<double2>(), n, p->batch, STEP,
+                               g->eterm.as<double>(), g->bh.misc.as<int>());
         prof.end();
         prof.begin(2);
         rc = launch_energy(g->eterm.as<double>(), n, g->energy.as<double>(), p->precision, s);

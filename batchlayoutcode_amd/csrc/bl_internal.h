@@ -120,6 +120,41 @@ struct Prof {
     }
 };
 
+// Neighbours that the reference's merge pointer (algorithms.cpp:758-762) meets, for rows
+// [i0, i1) of a CSR whose entries end at `end` (the array end of that graph): the pointer
+// starts at rowptr[i], matches when j == colids[ptr] and advances only while
+// ptr < rowptr[i+1]-1, so with j ascending the matched set is the longest strictly
+// increasing prefix of the row; an empty row keeps the pointer on the next row's first
+// entry, which then acts as one neighbour (an empty row at the end meets nothing).
+// Appends to eci (column ids), ew (weights, when w is given) and sets erp[i+1] for each row
+// (erp[i0] must hold the running count already).
+inline void effective_neighbors(const int32_t *rowptr, const int32_t *colids, const float *w, int32_t i0, int32_t i1,
+                                int64_t end, std::vector<int32_t> &erp, std::vector<int32_t> &eci,
+                                std::vector<float> &ew, int32_t *max_deg) {
+    for (int32_t i = i0; i < i1; ++i) {
+        const int64_t s = rowptr[i], e = rowptr[i + 1];
+        int32_t cnt = 0;
+        if (s < e) {
+            int32_t last = colids[s];
+            eci.push_back(last);
+            if (w) ew.push_back(w[s]);
+            ++cnt;
+            for (int64_t k = s + 1; k < e && colids[k] > last; ++k) {
+                last = colids[k];
+                eci.push_back(last);
+                if (w) ew.push_back(w[k]);
+                ++cnt;
+            }
+        } else if (s < end) {
+            eci.push_back(colids[s]);
+            if (w) ew.push_back(w[s]);
+            ++cnt;
+        }
+        erp[i + 1] = erp[i] + cnt;
+        if (max_deg && cnt > *max_deg) *max_deg = cnt;
+    }
+}
+
 // minibatch.hip
 int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0,
                   int iters, bool converged, bl_stats *st);
@@ -133,6 +168,7 @@ int bh_forces_host(int32_t n, const double *xy, double theta, int32_t nq, const 
                    int32_t precision);
 // batched.hip
 int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,
-                const int32_t *colids, double *xy, const bl_params *p, double *energies, bl_stats *st);
+                const int32_t *colids, double *xy, const bl_params *p, double *energies, double *finish_ms,
+                bl_stats *st);
 
 }  // namespace bl

@@ -38,85 +38,23 @@
 #include <algorithm>
 
 #include "bl_internal.h"
+#include "pair_math.h"
 
 namespace bl {
 namespace {
 
 constexpr int WG = 256;
-// |d|^2 floor of the FP32 path: the self pair and coincident vertices give
-// 0 * (1/kTiny) = 0, as the reference's `dist > 0` test skips them.
-constexpr float kTiny = 1e-30f;
 constexpr int kBlockY = 512;  // the reference's j block, algorithms.cpp:717
 constexpr int UNROLL = 16;
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Attraction of one neighbour pair, in the reference's FP64 operation order:
-// f += d * (W * |d|) (:759), f += d * W (:985), f += d * log2(1 + W|d|) (:912).
-__device__ __forceinline__ void attract64(int law, double W, double dx, double dy, double &fx, double &fy) {
-    double s;
-    if (law == LAW_FA) {
-        s = W;
-    } else {
-        const double mag = sqrt(dx * dx + dy * dy);
-        s = (law == LAW_FR) ? W * mag : log2(1.0 + W * mag);
-    }
-    fx += dx * s;
-    fy += dy * s;
-}
 
 // Read-only view of the positions in the constant address space: with a wave-uniform
 // index the compiler fetches x_j with scalar loads (one fetch per wave, no VGPRs).  Only
 // used by kernels during which the positions do not change.
 typedef const __attribute__((address_space(4))) double cdouble;
 
-// 1/|d|^2 for a pair with FP64 difference (dx, dy).  REFINE: |d|^2 in FP64, the FP32
-// reciprocal as seed and one FP64 Newton step (relative error ~1e-14).  Otherwise |d|^2 and
-// the reciprocal in FP32 (relative error ~1e-7).  kTiny keeps the self pair at 0 * finite.
-template <bool REFINE>
-__device__ __forceinline__ double inv_r2(const double dx, const double dy) {
-    if constexpr (REFINE) {
-        const double r2 = fma(dy, dy, fma(dx, dx, (double)kTiny));
-        const double s = (double)__builtin_amdgcn_rcpf((float)r2);
-        return fma(s, fma(-r2, s, 1.0), s);
-    } else {
-        const float fx = (float)dx, fy = (float)dy;
-        const float r2 = fmaf(fy, fy, fmaf(fx, fx, kTiny));
-        return (double)__builtin_amdgcn_rcpf(r2);
-    }
-}
-
-// One repulsion pair of the FP32/mixed paths: -d/|d|^2 accumulated in FP64.
-template <bool REFINE>
-__device__ __forceinline__ void pair32(const double qx, const double qy, const double px, const double py, double &ax,
-                                       double &ay) {
-    const double dx = qx - px, dy = qy - py;
-    const double inv = inv_r2<REFINE>(dx, dy);
-    ax = fma(-dx, inv, ax);
-    ay = fma(-dy, inv, ay);
-}
-
-// Neighbour correction of the FP32/mixed paths for one (i, j) pair: removes the product the
-// dense loop accumulated for it (dx * inv with the same operands) and adds the attraction.
-template <bool REFINE>
-__device__ __forceinline__ void correct32(int law, double W, double dx, double dy, double &cx, double &cy) {
-    const double inv = inv_r2<REFINE>(dx, dy);
-    cx += dx * inv;
-    cy += dy * inv;
-    attract64(law, W, dx, dy, cx, cy);
-}
-
 // x_i += (F/|F|) * STEP and |F|^2, algorithms.cpp:775-779 / Coordinate.h:57-65.
 __device__ __forceinline__ void move_one(double2 *pos, int i, double fx, double fy, double step, double *eterm) {
-    const double mag = sqrt(fx * fx + fy * fy);
-    double2 p = pos[i];
-    p.x = p.x + (fx / mag) * step;
-    p.y = p.y + (fy / mag) * step;
-    pos[i] = p;
+    pos[i] = moved(pos[i], fx, fy, step);
     eterm[i] = fx * fx + fy * fy;
 }
 
@@ -191,12 +129,12 @@ __device__ __forceinline__ void dense_lds(const double2 *lds, int u0, int u1, do
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k) {
             const double2 q = lds[u + k];
-            pair32<true>(q.x, q.y, px, py, ax, ay);
+            pair_rep<true>(q.x, q.y, px, py, ax, ay);
         }
     }
     for (; u < u1; ++u) {
         const double2 q = lds[u];
-        pair32<true>(q.x, q.y, px, py, ax, ay);
+        pair_rep<true>(q.x, q.y, px, py, ax, ay);
     }
 }
 
@@ -263,7 +201,7 @@ __global__ __launch_bounds__(WG) void k_iter32(IterArgs a) {
             const int kend = a.erp[i + 1];
             for (int e = a.erp[i] + lane; e < kend; e += 64) {
                 const double2 pj = a.pos[a.eci[e]];
-                correct32<true>(a.law, a.ew ? a.W * (double)a.ew[e] : a.W, pj.x - pi.x, pj.y - pi.y, cx, cy);
+                correct_pair<true>(a.law, a.ew ? a.W * (double)a.ew[e] : a.W, pj.x - pi.x, pj.y - pi.y, cx, cy);
             }
             cx = wave_sum(cx);
             cy = wave_sum(cy);
@@ -325,7 +263,7 @@ __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos,
         const int kend = erp[i + 1];
         for (int k = erp[i] + lane; k < kend; k += 64) {
             const double2 pj = pos[eci[k]];
-            correct32<REFINE>(law, ew ? W * (double)ew[k] : W, pj.x - pi.x, pj.y - pi.y, cx, cy);
+            correct_pair<REFINE>(law, ew ? W * (double)ew[k] : W, pj.x - pi.x, pj.y - pi.y, cx, cy);
         }
         cx = wave_sum(cx);
         cy = wave_sum(cy);
@@ -348,9 +286,9 @@ __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos,
     for (; u + UNROLL <= j1; u += UNROLL) {
 #pragma unroll
         for (int k = 0; k < UNROLL; ++k)
-            pair32<REFINE>(cpos[2 * (u + k)], cpos[2 * (u + k) + 1], px, py, ax, ay);
+            pair_rep<REFINE>(cpos[2 * (u + k)], cpos[2 * (u + k) + 1], px, py, ax, ay);
     }
-    for (; u < j1; ++u) pair32<REFINE>(cpos[2 * u], cpos[2 * u + 1], px, py, ax, ay);
+    for (; u < j1; ++u) pair_rep<REFINE>(cpos[2 * u], cpos[2 * u + 1], px, py, ax, ay);
     if constexpr (JS > 1) {
         red[tid] = make_double2(ax, ay);
         __syncthreads();

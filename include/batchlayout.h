@@ -144,15 +144,17 @@ int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t nq, const do
 
 /* ---- batched small graphs -------------------------------------------------------- */
 /* Lays out `count` independent graphs in one persistent launch (no counterpart in the
- * reference, which lays out one graph per process).  Graph k has vertices
- * [voff[k], voff[k+1]) and CSR entries [eoff[k], eoff[k+1]) of the concatenated arrays;
- * rowptr is global (rowptr[voff[k]] == eoff[k]) and colids are local to the graph.
- * Every graph runs algo p->algo (2 or 8) with p->batch/iters/weight; energies[k] receives
- * each graph's final energy.  Graphs up to BL_BATCHED_MAX_N vertices. */
+ * reference, which lays out one graph per process; each graph follows the same algorithm as
+ * bl_run).  Graph k has vertices [voff[k], voff[k+1]) and CSR entries [eoff[k], eoff[k+1])
+ * of the concatenated arrays; rowptr is global (rowptr[voff[k]] == eoff[k]) and colids are
+ * local to the graph.  Every graph runs p->algo (2 or 8) with p->batch/iters/weight in
+ * precision BL_PREC_MIXED or BL_PREC_FP32; energies[k] (nullable) receives each graph's final
+ * energy and finish_ms[k] (nullable) the time from the launch start to the graph's
+ * completion.  Graphs of up to BL_BATCHED_MAX_N vertices. */
 #define BL_BATCHED_MAX_N 8192
 int bl_run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,
                    const int32_t *colids, double *xy_inout, const bl_params *p, double *energies,
-                   bl_stats *st);
+                   double *finish_ms, bl_stats *st);
 
 /* Library/device information: writes a short NUL-terminated description. */
 int bl_device_info(char *buf, int32_t len);
