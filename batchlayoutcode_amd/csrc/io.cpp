@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batchlayout.h"
@@ -66,6 +67,57 @@ inline bool parse_long(const char *&p, const char *end, long &v) {
     return true;
 }
 
+// Entries of one chunk of the .mtx body: 0-based (row, col) pairs, interleaved.  `bad`
+// marks a malformed or out-of-range line, after which the chunk stops.
+struct Chunk {
+    std::vector<int32_t> rc;
+    bool bad = false;
+};
+
+void parse_chunk(const char *buf, size_t b, size_t e, long m, Chunk &out) {
+    out.rc.reserve((e - b) / 6 * 2 + 2);
+    size_t pos = b;
+    while (pos < e) {
+        const char *p = buf + pos;
+        const char *nl = (const char *)memchr(p, '\n', e - pos);
+        const char *pe = nl ? nl : buf + e;
+        pos = (size_t)(pe - buf) + (nl ? 1 : 0);
+        if (pe > p && pe[-1] == '\r') --pe;
+        while (p < pe && (*p == ' ' || *p == '\t')) ++p;
+        if (p == pe) continue;  // blank line
+        long ri, ci;
+        if (!parse_long(p, pe, ri) || !parse_long(p, pe, ci) || ri < 1 || ri > m || ci < 1 || ci > m) {
+            out.bad = true;
+            return;
+        }
+        out.rc.push_back((int32_t)(ri - 1));
+        out.rc.push_back((int32_t)(ci - 1));
+    }
+}
+
+// Threads for a body of `bytes` bytes: one per 4 MiB, at most the hardware threads and 16;
+// BL_IO_THREADS overrides.
+int io_threads(size_t bytes) {
+    int hw = (int)std::thread::hardware_concurrency();
+    if (hw <= 0) hw = 1;
+    int t = (int)std::min<size_t>(bytes / (4u << 20) + 1, (size_t)std::min(hw, 16));
+    if (const char *s = getenv("BL_IO_THREADS")) {
+        const int v = atoi(s);
+        if (v > 0) t = std::min(v, 64);
+    }
+    return std::max(t, 1);
+}
+
+template <class F>
+void run_threads(int T, F &&f) {
+    if (T == 1) { f(0); return; }
+    std::vector<std::thread> th;
+    th.reserve(T - 1);
+    for (int t = 1; t < T; ++t) th.emplace_back(f, t);
+    f(0);
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 extern "C" void bl_free(void *p) { free(p); }
@@ -90,40 +142,79 @@ extern "C" int bl_mtx_read(const char *path, int32_t *n_out, int64_t *nnz_out, i
         break;
     }
     if (m <= 0 || ncol != m || nnz < 0 || m > INT32_MAX) return BL_E_FORMAT;
-    std::vector<int32_t> rows, cols;
-    const size_t cap = (size_t)(symmetric ? 2 * nnz : nnz);
-    rows.reserve(cap);
-    cols.reserve(cap);
-    long lines = 0;
-    while (lines < nnz) {
-        if (r.eof()) return BL_E_FORMAT;  // fewer entries than declared (assert, IO.h:142)
-        r.line(b, e);
-        const char *p = r.buf.data() + b, *pe = r.buf.data() + e;
-        while (p < pe && (*p == ' ' || *p == '\t')) ++p;
-        if (p == pe) continue;  // blank line
-        long ri, ci;
-        if (!parse_long(p, pe, ri) || !parse_long(p, pe, ci)) return BL_E_FORMAT;
-        ++lines;
-        --ri;
-        --ci;
-        if (ri < 0 || ri >= m || ci < 0 || ci >= m) return BL_E_FORMAT;
-        if (symmetric) {
-            if (ri == ci) continue;
-            rows.push_back((int32_t)ri); cols.push_back((int32_t)ci);
-            rows.push_back((int32_t)ci); cols.push_back((int32_t)ri);
-        } else {
-            rows.push_back((int32_t)ri); cols.push_back((int32_t)ci);
-        }
+    if ((symmetric ? 2 * nnz : nnz) > INT32_MAX) return BL_E_FORMAT;  // CSR offsets are int32
+
+    // Entry lines are parsed by T threads, each over a newline-aligned chunk of the body.
+    // Only the first nnz non-blank lines count: a chunk records how many entries it parsed
+    // before a bad line, and the bad line is an error only if it falls inside the first nnz.
+    const size_t body = r.pos, total = r.buf.size();
+    const int T = io_threads(total - body);
+    std::vector<size_t> cut(T + 1, total);
+    cut[0] = body;
+    for (int t = 1; t < T; ++t) {
+        size_t c = std::max(cut[t - 1], body + (total - body) * (size_t)t / (size_t)T);
+        while (c < total && c > body && r.buf[c - 1] != '\n') ++c;
+        cut[t] = c;
     }
-    const size_t cnt = rows.size();
+    std::vector<Chunk> chunks(T);
+    run_threads(T, [&](int t) { parse_chunk(r.buf.data(), cut[t], cut[t + 1], m, chunks[t]); });
+    // take chunks in file order until nnz entries
+    long need = nnz;
+    std::vector<size_t> take(T, 0);
+    for (int t = 0; t < T && need > 0; ++t) {
+        const long have = (long)chunks[t].rc.size() / 2;
+        if (have >= need) { take[t] = (size_t)need; need = 0; break; }
+        if (chunks[t].bad) return BL_E_FORMAT;  // malformed or out-of-range line within nnz
+        take[t] = (size_t)have;
+        need -= have;
+    }
+    if (need > 0) return BL_E_FORMAT;  // fewer entries than declared (assert, IO.h:142)
+
     int32_t *rowptr = (int32_t *)calloc((size_t)m + 1, sizeof(int32_t));
-    int32_t *colids = (int32_t *)malloc(sizeof(int32_t) * (cnt > 0 ? cnt : 1));
-    if (!rowptr || !colids) { free(rowptr); free(colids); return BL_E_NOMEM; }
-    for (size_t k = 0; k < cnt; ++k) rowptr[rows[k] + 1]++;
+    if (!rowptr) return BL_E_NOMEM;
+    // row counts (rowptr[i+1]) with relaxed atomics; symmetric drops diagonals and mirrors
+    run_threads(T, [&](int t) {
+        const int32_t *rc = chunks[t].rc.data();
+        for (size_t k = 0; k < take[t]; ++k) {
+            const int32_t ri = rc[2 * k], ci = rc[2 * k + 1];
+            if (symmetric) {
+                if (ri == ci) continue;
+                __atomic_fetch_add(&rowptr[ci + 1], 1, __ATOMIC_RELAXED);
+            }
+            __atomic_fetch_add(&rowptr[ri + 1], 1, __ATOMIC_RELAXED);
+        }
+    });
     for (long i = 0; i < m; ++i) rowptr[i + 1] += rowptr[i];
-    std::vector<int32_t> fill(rowptr, rowptr + m);
-    for (size_t k = 0; k < cnt; ++k) colids[fill[rows[k]]++] = cols[k];
-    for (long i = 0; i < m; ++i) std::sort(colids + rowptr[i], colids + rowptr[i + 1]);
+    const size_t cnt = (size_t)rowptr[m];
+    int32_t *colids = (int32_t *)malloc(sizeof(int32_t) * (cnt > 0 ? cnt : 1));
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)m);
+    if (!colids || !fill) { free(rowptr); free(colids); free(fill); return BL_E_NOMEM; }
+    memcpy(fill, rowptr, sizeof(int32_t) * (size_t)m);
+    // scatter in any order: each row is sorted afterwards, and equal column ids are
+    // indistinguishable, so the result is the sequential one
+    run_threads(T, [&](int t) {
+        const int32_t *rc = chunks[t].rc.data();
+        for (size_t k = 0; k < take[t]; ++k) {
+            const int32_t ri = rc[2 * k], ci = rc[2 * k + 1];
+            if (symmetric) {
+                if (ri == ci) continue;
+                colids[__atomic_fetch_add(&fill[ci], 1, __ATOMIC_RELAXED)] = ri;
+            }
+            colids[__atomic_fetch_add(&fill[ri], 1, __ATOMIC_RELAXED)] = ci;
+        }
+    });
+    free(fill);
+    chunks.clear();
+    // sort rows; threads take contiguous row ranges of about cnt/T entries each
+    std::vector<long> rcut(T + 1, m);
+    rcut[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        const int32_t target = (int32_t)(cnt * (size_t)t / (size_t)T);
+        rcut[t] = std::max(rcut[t - 1], (long)(std::lower_bound(rowptr, rowptr + m, target) - rowptr));
+    }
+    run_threads(T, [&](int t) {
+        for (long i = rcut[t]; i < rcut[t + 1]; ++i) std::sort(colids + rowptr[i], colids + rowptr[i + 1]);
+    });
     *n_out = (int32_t)m;
     *nnz_out = (int64_t)cnt;
     *rowptr_out = rowptr;

@@ -48,6 +48,40 @@ def test_mtx_read_matches_oracle_on_random_files(bl, oracle, tmp_path):
         assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
 
 
+@pytest.mark.parametrize("threads", ["1", "3", "16"])
+def test_mtx_read_threaded_chunks(bl, oracle, tmp_path, monkeypatch, threads):
+    # a body large enough to split into chunks, with CRLF ends, tabs and values; lines after
+    # the declared count (including a malformed one) are never read
+    rng = np.random.default_rng(11)
+    n, m = 3000, 60000
+    e = rng.integers(1, n + 1, size=(m, 2))
+
+    def write(path, blank, declared):
+        with open(path, "w", newline="") as f:
+            f.write("%%MatrixMarket matrix coordinate real symmetric\r\n% c\r\n" + f"{n} {n} {declared}\r\n")
+            for k, (a, b) in enumerate(e):
+                if blank and k % 997 == 0:
+                    f.write("\r\n   \n")
+                f.write(f"{a}\t {b} {k % 7}.5\r\n" if k % 5 == 0 else f"{a} {b}\n")
+            f.write("1 x\nnot an entry\n")
+
+    p, pb, q = tmp_path / "big.mtx", tmp_path / "blank.mtx", tmp_path / "bad.mtx"
+    write(p, False, m)
+    write(pb, True, m)
+    # declaring one entry more than the file holds reaches the malformed line
+    write(q, True, m + 1)
+    monkeypatch.setenv("BL_IO_THREADS", threads)
+    r1 = bl.mtx_read(str(p))
+    r2 = oracle.mtx_read(str(p))
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+    # blank lines (where the reference would fail) are skipped
+    r3 = bl.mtx_read(str(pb))
+    assert np.array_equal(r1[0], r3[0]) and np.array_equal(r1[1], r3[1])
+    with pytest.raises(bl.BLError) as ei:
+        bl.mtx_read(str(q))
+    assert ei.value.code == -3
+
+
 def test_mtx_read_errors(bl, tmp_path):
     cases = {
         "short.mtx": ("%%MatrixMarket matrix coordinate pattern symmetric\n4 4 3\n2 1\n3 2\n", -3),
