@@ -47,7 +47,8 @@ struct BatchedArgs {
 template <bool REFINE>
 __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *red, double2 *fb, double *eb) {
     const int tid = threadIdx.x;
-    const int v0 = a.voff[g], n = a.voff[g + 1] - v0;
+    const int v0 = __builtin_amdgcn_readfirstlane(a.voff[g]);
+    const int n = __builtin_amdgcn_readfirstlane(a.voff[g + 1]) - v0;
     const bool in_lds = n <= a.lds_cap;
     double2 *gp = in_lds ? P : a.pos + v0;
     if (in_lds) {
@@ -115,11 +116,11 @@ __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *r
     if (in_lds) {
         for (int k = tid; k < n; k += BT) a.pos[v0 + k] = P[k];
     }
+    // every thread holds the same E (batch sums come from eb[0]); all of them store it so that
+    // no per-lane branch sits next to the queue loop's back edge
+    a.energies[g] = E;
+    a.finish[g] = wall_clock64();
     __syncthreads();
-    if (tid == 0) {
-        a.energies[g] = E;
-        a.finish[g] = wall_clock64();
-    }
 }
 
 __global__ __launch_bounds__(BT) void k_batched(BatchedArgs a) {
@@ -133,10 +134,14 @@ __global__ __launch_bounds__(BT) void k_batched(BatchedArgs a) {
     for (;;) {
         if (tid == 0) slot = atomicAdd(a.queue, 1);
         __syncthreads();
-        const int q = slot;
+        // The slot is the same for every thread; readfirstlane makes that visible to the
+        // compiler.  Left as a per-lane LDS value, the loop exit looks divergent and the
+        // loop may be restructured so that lane 0 (which advances the queue) leaves the
+        // per-graph loop while the other lanes of its wave repeat the same graph forever.
+        const int q = __builtin_amdgcn_readfirstlane(slot);
         __syncthreads();
         if (q >= a.count) break;
-        const int g = a.order[q];
+        const int g = __builtin_amdgcn_readfirstlane(a.order[q]);
         if (a.refine)
             layout_graph<true>(a, g, P, red, fb, eb);
         else
