@@ -18,6 +18,9 @@ BL_OK = 0
 STATUS = {0: "BL_OK", -1: "BL_E_ARG", -2: "BL_E_IO", -3: "BL_E_FORMAT", -4: "BL_E_DEVICE",
           -5: "BL_E_DUPKEY", -6: "BL_E_NOMEM", -7: "BL_E_GRAPH"}
 ALGO_CACHE, ALGO_LINLOG, ALGO_BH, ALGO_BHCACHE, ALGO_FA = 2, 3, 4, 6, 8
+# physically consistent Barnes-Hut layouts (no reference counterpart)
+ALGO_FAR_MONO, ALGO_FAR_EXACT = 40, 41
+FAR_MONO, FAR_EXACT = 1, 2
 PREC_MIXED, PREC_FP64, PREC_FP32 = 0, 1, 2
 BATCHED_MAX_N = 8192
 
@@ -70,6 +73,7 @@ def lib():
         L.bl_bh_tree.argtypes = [C.c_int32, _f64p, _i32p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  _f64p, _f64p, _f64p, _i32p]
         L.bl_bh_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, _f64p, _f64p, C.c_int32]
+        L.bl_far_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, C.c_int32, _f64p, _f64p]
         L.bl_run_batched.argtypes = [C.c_int32, _i32p, _i64p, _i32p, _i32p, _f64p, C.POINTER(bl_params), _f64p,
                                      _f64p, C.POINTER(bl_stats)]
         L.bl_device_info.argtypes = [C.c_char_p, C.c_int32]
@@ -213,6 +217,15 @@ def bh_forces(xy, theta, queries=None, precision=PREC_FP32):
     q = xy if queries is None else np.ascontiguousarray(queries, dtype=np.float64)
     out = np.zeros((len(q), 2))
     _chk(lib().bl_bh_forces(len(xy), xy, theta, len(q), q, out, precision), "bl_bh_forces")
+    return out
+
+
+def far_forces(xy, theta, mode=FAR_MONO, queries=None):
+    """Repulsion sum_j (x_j - q)/|x_j - q|^2 by mass-weighted Barnes-Hut (FAR_MONO) or exactly."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    q = xy if queries is None else np.ascontiguousarray(queries, dtype=np.float64)
+    out = np.zeros((len(q), 2))
+    _chk(lib().bl_far_forces(len(xy), xy, theta, mode, len(q), q, out), "bl_far_forces")
     return out
 
 

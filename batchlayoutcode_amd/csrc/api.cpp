@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "bl_internal.h"
+#include "bh_common.h"
 
 namespace bl {
 
@@ -42,7 +42,7 @@ void DevBuf::release() {
 void BHWork::release() {
     DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &hist, &spos, &frame, &isrep, &nodeid, &bnd_lo,
                      &node_child, &node_depth, &node_cnt, &node_parent, &node_cxy, &node_rep, &scan_tmp,
-                     &rep, &misc};
+                     &rep, &misc, &far_cells, &far_arrive};
     for (DevBuf *b : all) b->release();
     n_alloc = 0;
 }
@@ -172,6 +172,7 @@ static int validate_params(const bl_params *p) {
         return BL_E_ARG;
     switch (p->algo) {
         case BL_ALGO_CACHE: case BL_ALGO_LINLOG: case BL_ALGO_BH: case BL_ALGO_BHCACHE: case BL_ALGO_FA:
+        case BL_ALGO_FAR_MONO: case BL_ALGO_FAR_EXACT:
             return BL_OK;
         default:
             return BL_E_ARG;
@@ -209,6 +210,12 @@ static int run_on_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats
             }
             break;
         }
+        case BL_ALGO_FAR_MONO:
+            rc = run_bh(g, d_xy, p, p->step0, p->iters, p->bht, &local, FAR_MONO);
+            break;
+        case BL_ALGO_FAR_EXACT:
+            rc = run_bh(g, d_xy, p, p->step0, p->iters, 0.0, &local, FAR_EXACT);
+            break;
         default:
             rc = BL_E_ARG;
     }
@@ -278,6 +285,15 @@ extern "C" int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t n
     int rc = check_device_once();
     if (rc) return rc;
     return bh_forces_host(n, xy, theta, nq, q, out, precision);
+}
+
+extern "C" int bl_far_forces(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
+                             double *out) {
+    if (n <= 0 || !xy || nq < 0 || (nq > 0 && (!q || !out)) || theta < 0.0) return BL_E_ARG;
+    if (mode != BL_FAR_MONO && mode != BL_FAR_EXACT) return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return far_forces_host(n, xy, theta, mode == BL_FAR_MONO ? FAR_MONO : FAR_EXACT, nq, q, out);
 }
 
 extern "C" int bl_run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,

@@ -38,7 +38,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "bl_internal.h"
+#include "bh_common.h"
 
 namespace bl {
 
@@ -53,17 +53,7 @@ constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = WG * SCAN_ITEMS;     // 2048 entries per workgroup
 constexpr int STACK = 128;                     // per-wave traversal stack (depth <= 32 -> <= 100 entries)
 
-struct BHNode {
-    double2 c;     // centroid
-    int4 ch;       // children: >= 0 node, -1 empty, <= -2 leaf at Morton rank (-c-2)
-    int depth;     // base-4 level of the cell: diameter = MAXMIN * 2^-depth
-    int nint;      // number of internal children
-    int parent;
-    int cnt;       // bottom-up arrival counter
-};
 
-// status word bits
-constexpr int ST_DUPKEY = 1;
 
 // ---------------------------------------------------------------------------------------
 // 1. bounding box
@@ -654,28 +644,77 @@ __global__ __launch_bounds__(1024) void k_attract_move(const int *__restrict__ r
     }
 }
 
-// Same batch walk for the FP32/mixed modes with the attraction work of each batch split
+// Same batch walk for the FP32/mixed modes, with the attraction work of each batch split
 // evenly over the 1024 threads by CSR entry (a hub row of tens of thousands of entries is
 // shared by many threads instead of one).  Thread t takes the entries [E0 + t*c, E0 + t*c + c)
 // of the batch; a row lying inside one thread's range is summed by that thread; a row spread
 // over several threads is the sum, in thread order, of those threads' partial sums (the
 // range's last row of the first thread, then whole ranges, then the first row of the last
 // thread), so the result does not depend on timing.
-__global__ __launch_bounds__(1024) void k_attract_move_bal(const int *__restrict__ rowptr,
-                                                           const int *__restrict__ colids,
-                                                           const float *__restrict__ w_edge, double W,
-                                                           double2 *__restrict__ pos, const double2 *__restrict__ rep,
-                                                           double2 *__restrict__ fbuf, int n, int B, double step,
-                                                           double *__restrict__ eterm, const int *__restrict__ status) {
-    constexpr int T = 1024;
+//
+// Latency: the walk is one dependent step per batch, so every global load on a step's path
+// costs a full memory round trip.  With STAGE (batches of at most WALK_MAXB vertices) the
+// batch's rowptr slice, own positions and repulsion sit in LDS, loaded one batch ahead (they
+// do not depend on the moves of the current batch: a batch moves only its own vertices);
+// row lookups are binary searches in LDS; the entries are fetched WALK_G at a time so their
+// colids and neighbour-position loads overlap; the force of each batch vertex is combined and
+// applied by the same thread, without a round trip through global memory.
+constexpr int WALK_T = 1024;
+constexpr int WALK_G = 8;
+constexpr int WALK_MAXB = 2048;
+
+template <bool STAGE>
+__global__ __launch_bounds__(WALK_T) void k_walk(const int *__restrict__ rowptr, const int *__restrict__ colids,
+                                                  const float *__restrict__ w_edge, double W,
+                                                  double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                  double2 *__restrict__ fbuf, int n, int B, double step,
+                                                  double *__restrict__ eterm, const int *__restrict__ status) {
+    constexpr int T = WALK_T;
+    constexpr int SB = STAGE ? WALK_MAXB : 1;
+    constexpr int PER = (WALK_MAXB + T - 1) / T;  // staged vertices per thread
     __shared__ double2 head[T], tail[T];
     __shared__ int hrow[T], trow[T];
+    __shared__ int rps[SB + 1];
+    __shared__ double2 bpos[SB], rsum[SB];
     if (*status) return;
     const int t = threadIdx.x;
     const int nb = (n + B - 1) / B;
+    // staged data of the next batch, held in registers until the current batch is done
+    int nrp[PER + 1];
+    double2 npos[PER], nrep[PER];
+    auto fetch = [&](int b) {
+        const int b0 = b * B, bc = min(n, b0 + B) - b0;
+#pragma unroll
+        for (int j = 0; j <= PER; ++j) {
+            const int k = t + j * T;
+            nrp[j] = k <= bc ? rowptr[b0 + k] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int k = t + j * T;
+            npos[j] = k < bc ? pos[b0 + k] : make_double2(0.0, 0.0);
+            nrep[j] = k < bc ? rep[b0 + k] : make_double2(0.0, 0.0);
+        }
+    };
+    if (STAGE) fetch(0);
     for (int b = 0; b < nb; ++b) {
-        const int b0 = b * B, b1 = min(n, b0 + B);
-        const int E0 = rowptr[b0], E1 = rowptr[b1];
+        const int b0 = b * B, b1 = min(n, b0 + B), bc = b1 - b0;
+        if (STAGE) {
+#pragma unroll
+            for (int j = 0; j <= PER; ++j) {
+                const int k = t + j * T;
+                if (k <= bc) rps[k] = nrp[j];
+            }
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int k = t + j * T;
+                if (k < bc) bpos[k] = npos[j];
+            }
+            __syncthreads();
+        }
+        auto RP = [&](int i) { return STAGE ? rps[i - b0] : rowptr[i]; };  // i in [b0, b1]
+        auto OWN = [&](int i) { return STAGE ? bpos[i - b0] : pos[i]; };   // i in [b0, b1)
+        const int E0 = RP(b0), E1 = RP(b1);
         const int total = E1 - E0;
         const int c = max(1, (total + T - 1) / T);
         const int s = E0 + t * c, e = min(E1, s + c);
@@ -685,32 +724,47 @@ __global__ __launch_bounds__(1024) void k_attract_move_bal(const int *__restrict
             int lo = b0, hi = b1;  // row containing entry s: last r with rowptr[r] <= s
             while (hi - lo > 1) {
                 const int m = (lo + hi) >> 1;
-                if (rowptr[m] <= s) lo = m; else hi = m;
+                if (RP(m) <= s) lo = m; else hi = m;
             }
             int r = lo;
             const int first = r;
-            int rend = rowptr[r + 1];
-            double2 pi = pos[r];
+            int rend = RP(r + 1);
+            double2 pi = OWN(r);
             double ax = 0.0, ay = 0.0;
-            for (int k = s; k < e; ++k) {
-                while (k >= rend) {  // row r complete inside this range
-                    if (r == first) {
-                        hrow[t] = r;
-                        head[t] = make_double2(ax, ay);
-                    } else {
-                        fbuf[r] = make_double2(ax, ay);
-                    }
-                    ++r;
-                    rend = rowptr[r + 1];
-                    pi = pos[r];
-                    ax = ay = 0.0;
+            for (int k0 = s; k0 < e; k0 += WALK_G) {
+                int cid[WALK_G];
+                double2 pj[WALK_G];
+                double wk[WALK_G];
+#pragma unroll
+                for (int g = 0; g < WALK_G; ++g) cid[g] = k0 + g < e ? colids[k0 + g] : -1;
+#pragma unroll
+                for (int g = 0; g < WALK_G; ++g) {
+                    pj[g] = cid[g] >= 0 ? pos[cid[g]] : make_double2(0.0, 0.0);
+                    wk[g] = (w_edge && k0 + g < e) ? W * (double)w_edge[k0 + g] : W;
                 }
-                const double2 pj = pos[colids[k]];
-                const double dx = pj.x - pi.x, dy = pj.y - pi.y;
-                const double Wk = w_edge ? W * (double)w_edge[k] : W;
-                const double sc = Wk * sqrt(dx * dx + dy * dy);
-                ax = fma(dx, sc, ax);
-                ay = fma(dy, sc, ay);
+#pragma unroll
+                for (int g = 0; g < WALK_G; ++g) {
+                    const int k = k0 + g;
+                    if (k >= e) break;
+                    while (k >= rend) {  // row r complete inside this range
+                        if (r == first) {
+                            hrow[t] = r;
+                            head[t] = make_double2(ax, ay);
+                        } else if (STAGE) {
+                            rsum[r - b0] = make_double2(ax, ay);
+                        } else {
+                            fbuf[r] = make_double2(ax, ay);
+                        }
+                        ++r;
+                        rend = RP(r + 1);
+                        pi = OWN(r);
+                        ax = ay = 0.0;
+                    }
+                    const double dx = pj[g].x - pi.x, dy = pj[g].y - pi.y;
+                    const double sc = wk[g] * sqrt(dx * dx + dy * dy);
+                    ax = fma(dx, sc, ax);
+                    ay = fma(dy, sc, ay);
+                }
             }
             if (r == first) {
                 hrow[t] = r;
@@ -721,13 +775,21 @@ __global__ __launch_bounds__(1024) void k_attract_move_bal(const int *__restrict
             }
         }
         __syncthreads();
-        for (int i = b0 + t; i < b1; i += T) {
-            const int rs = rowptr[i], re = rowptr[i + 1];
+        // the next batch's staged data: issued now so that it arrives while this batch is
+        // combined and moved
+        double2 cur_rep[PER];
+        if (STAGE) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) cur_rep[j] = nrep[j];
+            if (b + 1 < nb) fetch(b + 1);
+        }
+        for (int i = b0 + t, j = 0; i < b1; i += T, ++j) {
+            const int rs = RP(i), re = RP(i + 1);
             double2 f = make_double2(0.0, 0.0);
             if (re > rs) {
                 const int ta = (rs - E0) / c, tb = (re - 1 - E0) / c;
                 if (ta == tb) {
-                    f = hrow[ta] == i ? head[ta] : (trow[ta] == i ? tail[ta] : fbuf[i]);
+                    f = hrow[ta] == i ? head[ta] : (trow[ta] == i ? tail[ta] : (STAGE ? rsum[i - b0] : fbuf[i]));
                 } else {
                     f = trow[ta] == i ? tail[ta] : head[ta];
                     for (int q = ta + 1; q <= tb; ++q) {
@@ -736,14 +798,19 @@ __global__ __launch_bounds__(1024) void k_attract_move_bal(const int *__restrict
                     }
                 }
             }
-            const double2 rr = rep[i];
-            fbuf[i] = make_double2(f.x - rr.x, f.y - rr.y);
-        }
-        __syncthreads();
-        for (int i = b0 + t; i < b1; i += T) {
-            const double2 f = fbuf[i];
+            double2 rr;
+            if (STAGE) {
+                rr = cur_rep[0];
+#pragma unroll
+                for (int u = 1; u < PER; ++u)
+                    if (j == u) rr = cur_rep[u];
+            } else {
+                rr = rep[i];
+            }
+            f.x -= rr.x;  // f - calcRepForce(x_i), :1059
+            f.y -= rr.y;
             const double mag = sqrt(f.x * f.x + f.y * f.y);
-            double2 q = pos[i];
+            double2 q = OWN(i);
             q.x = q.x + (f.x / mag) * step;
             q.y = q.y + (f.y / mag) * step;
             pos[i] = q;
@@ -857,7 +924,12 @@ int launch_rep(BHWork &w, int n, double theta, int precision, hipStream_t s) {
 
 }  // namespace
 
-int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta, bl_stats *st) {
+int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s) {
+    return build_tree(w, d_pos, n, s, nullptr, nullptr);
+}
+
+int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta, bl_stats *st,
+           int far_mode) {
     const int n = g->n;
     hipStream_t s = g->stream;
     double2 *pos = reinterpret_cast<double2 *>(d_xy);
@@ -871,12 +943,15 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     prof.s = s;
     double STEP = step0;
     for (int it = 0; it < iters; ++it) {
-        prof.begin(3);
-        rc = build_tree(g->bh, pos, n, s, nullptr, nullptr);  // tree of the iteration-start positions
-        prof.end();
-        if (rc) return rc;
+        if (far_mode != FAR_EXACT) {
+            prof.begin(3);
+            rc = build_tree(g->bh, pos, n, s, nullptr, nullptr);  // tree of the iteration-start positions
+            prof.end();
+            if (rc) return rc;
+        }
         prof.begin(0);
-        rc = launch_rep(g->bh, n, theta, p->precision, s);
+        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, far_mode, nullptr, s)
+                      : launch_rep(g->bh, n, theta, p->precision, s);
         prof.end();
         if (rc) return rc;
         prof.begin(1);
@@ -886,10 +961,10 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
                                g->bh.misc.as<int>());
         else
-            hipLaunchKernelGGL(k_attract_move_bal, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(),
-                               g->colids.as<int>(), g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos,
-                               g->bh.rep.as<double2>(), g->scratch.as<double2>(), n, p->batch, STEP,
-                               g->eterm.as<double>(), g->bh.misc.as<int>());
+            hipLaunchKernelGGL(p->batch <= WALK_MAXB ? k_walk<true> : k_walk<false>, dim3(1), dim3(WALK_T), 0, s,
+                               g->rowptr.as<int>(), g->colids.as<int>(), g->has_w ? g->w_edge.as<float>() : nullptr,
+                               p->weight, pos, g->bh.rep.as<double2>(), g->scratch.as<double2>(), n, p->batch,
+                               STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
         prof.end();
         prof.begin(2);
         rc = launch_energy(g->eterm.as<double>(), n, g->energy.as<double>(), p->precision, s);

@@ -1,0 +1,35 @@
+// Quadtree layout shared by the Barnes-Hut translation units (bh.hip builds the tree;
+// farfield.hip reads it).
+#pragma once
+
+#include "bl_internal.h"
+
+namespace bl {
+
+// One node per cell where the Morton-sorted points first split (single-child chains of the
+// uncompressed quadtree collapsed into their lowest node).
+struct BHNode {
+    double2 c;     // unweighted centroid of the non-empty quadrants (BarnesHut.h:197-217)
+    int4 ch;       // children: >= 0 node, -1 empty, <= -2 leaf at Morton rank (-c-2)
+    int depth;     // base-4 level of the cell: side = MAXMIN * 2^-depth
+    int nint;      // number of internal children
+    int parent;
+    int cnt;       // bottom-up arrival counter
+};
+
+// status word bits (misc[0])
+constexpr int ST_DUPKEY = 1;
+
+// Builds the tree of the n points d_pos into w: w.keys_a / w.idx_a sorted keys and vertex
+// ids, w.spos positions in Morton order, w.node_child nodes, w.frame = {minX, minY, MAXMIN},
+// misc[1] = root node (-1 for a single point), misc[2] = node count.
+int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
+
+// Repulsion of every vertex for the physically consistent layouts (farfield.hip): into
+// w.rep (by vertex id).  FAR_MONO and FAR_LEARNED read the tree of d_pos already built in w
+// by bh_build_tree; FAR_EXACT sums over all points.
+enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3 };
+int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode, const float *model,
+                  hipStream_t s);
+
+}  // namespace bl

@@ -42,6 +42,7 @@ struct BHWork {
     DevBuf scan_tmp;
     DevBuf rep;                              // repulsion result per vertex (double2)
     DevBuf misc;                             // counters / status words
+    DevBuf far_cells, far_arrive;            // physically consistent far field: cell summaries
     int32_t n_alloc = 0;
     void release();
 };
@@ -159,8 +160,12 @@ inline void effective_neighbors(const int32_t *rowptr, const int32_t *colids, co
 int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0,
                   int iters, bool converged, bl_stats *st);
 // bh.hip
+// far_mode 0: the reference's unit-vector law; else a FarMode of bh_common.h
 int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta,
-           bl_stats *st);
+           bl_stats *st, int far_mode = 0);
+// farfield.hip
+int far_forces_host(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
+                    double *out);
 int bh_morton_host(int32_t n, const double *xy, uint32_t *xs, uint32_t *ys, double *maxmin);
 int bh_tree_host(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, int32_t *nnodes,
                  int32_t *root, double *diam, double *cx, double *cy, int32_t *child);

@@ -43,6 +43,13 @@ extern "C" {
 #define BL_ALGO_BH 4         /* BarnesHutApproximation, :1020-1083 */
 #define BL_ALGO_BHCACHE 6    /* approxCacheBlockBH (80 % BH, then 20 % algo 2), :1192-1215 */
 #define BL_ALGO_FA 8         /* FAcacheBlockingminiBatchForceDirectedAlgorithm, :947-1018 */
+/* Physically consistent Barnes-Hut layouts (no reference counterpart).  The iteration of
+ * algo 4 (repulsion from the iteration-start positions, attraction d * W|d| over every CSR
+ * entry with live positions, sequential batches, STEP *= 0.999) with the repulsion law of
+ * algo 2 for every pair, d/|d|^2 (algorithms.cpp:764-768), instead of algo 4's unit vectors: */
+#define BL_ALGO_FAR_MONO 40  /* cells accepted by side^2 < bht^2 |c - v|^2 add their
+                                mass-weighted monopole m (c - v)/|c - v|^2 */
+#define BL_ALGO_FAR_EXACT 41 /* the exact sum over all vertices (O(n^2) per iteration) */
 
 /* ---- precision of the pair arithmetic -------------------------------------------- */
 #define BL_PREC_MIXED 0      /* default: FP64 differences and sums; 1/|d|^2 seeded by the FP32
@@ -141,6 +148,15 @@ int bl_bh_tree(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, in
  * unit vectors toward accepted cells and leaves of the tree of the n points xy. */
 int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t nq, const double *q,
                  double *out, int32_t precision);
+
+/* Repulsion of the physically consistent layouts (no reference counterpart) on the device:
+ * out[k] = approximation of sum_j (x_j - q_k)/|x_j - q_k|^2 over the n points xy.
+ * mode BL_FAR_MONO: mass-weighted monopoles of the cells with side^2 < theta^2 |c - q|^2 on
+ * the compressed quadtree of xy (single points exactly); BL_FAR_EXACT: the exact sum. */
+#define BL_FAR_MONO 1
+#define BL_FAR_EXACT 2
+int bl_far_forces(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
+                  double *out);
 
 /* ---- batched small graphs -------------------------------------------------------- */
 /* Lays out `count` independent graphs in one persistent launch (no counterpart in the

@@ -515,3 +515,236 @@ int orc_exact_repulsion(int32_t n, const double *xy, int32_t nq, const int32_t *
     }
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Physically consistent Barnes-Hut (no reference counterpart): the algo-2   */
+/* repulsion law d/|d|^2 with a mass-weighted monopole per accepted cell, on */
+/* the compressed quadtree of the Morton-sorted points (a cell exists where  */
+/* its points first split; its side is MAXMIN * 2^-d at base-4 digit d).     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const uint64_t *key; /* sorted keys */
+    const double *sx;    /* sorted coordinates */
+    double maxmin, theta2;
+    /* sample output (orc_far_samples) */
+    int32_t cap, cnt;
+    double *feat, *targ;
+    int32_t *cell;
+} far_t;
+
+/* first base-4 digit (0 = top) where the keys of [l, r) differ, r - l >= 2 */
+static int split_digit(const far_t *F, int32_t l, int32_t r) {
+    const uint64_t x = F->key[l] ^ F->key[r - 1];
+    return __builtin_clzll(x) / 2;
+}
+
+static int32_t lower_q(const far_t *F, int32_t l, int32_t r, int d, int q) {
+    while (l < r) {
+        int32_t m = l + (r - l) / 2;
+        int dig = (int)((F->key[m] >> (62 - 2 * d)) & 3u);
+        if (dig < q) l = m + 1; else r = m;
+    }
+    return l;
+}
+
+/* summary of the points [l, r): count, mean, central second moments (sums) */
+static void cell_summary(const far_t *F, int32_t l, int32_t r, double *cx, double *cy, double *mxx, double *mxy,
+                         double *myy) {
+    double sx = 0.0, sy = 0.0;
+    for (int32_t k = l; k < r; k++) { sx += F->sx[2 * k]; sy += F->sx[2 * k + 1]; }
+    const double m = (double)(r - l);
+    *cx = sx / m;
+    *cy = sy / m;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int32_t k = l; k < r; k++) {
+        const double ux = F->sx[2 * k] - *cx, uy = F->sx[2 * k + 1] - *cy;
+        a += ux * ux; b += ux * uy; c += uy * uy;
+    }
+    *mxx = a; *mxy = b; *myy = c;
+}
+
+/* learned far-field features of an accepted cell seen from v (rotated into the frame of
+ * u = (c - v)/|c - v|, w = u rotated by +90 degrees), and the targets (a, b) with
+ * F_exact = (m/rho) * ((1 + a) u + b w). */
+static void far_features(double m, double D, double rx, double ry, double mxx, double mxy, double myy, double *f) {
+    const double rho = sqrt(rx * rx + ry * ry);
+    const double ux = rx / rho, uy = ry / rho, wx = -uy, wy = ux;
+    const double s = 1.0 / (m * rho * rho);
+    const double suu = (ux * ux * mxx + 2.0 * ux * uy * mxy + uy * uy * myy) * s;
+    const double suw = (ux * wx * mxx + (ux * wy + uy * wx) * mxy + uy * wy * myy) * s;
+    const double sww = (wx * wx * mxx + 2.0 * wx * wy * mxy + wy * wy * myy) * s;
+    f[0] = D / rho;
+    f[1] = suu;
+    f[2] = suw;
+    f[3] = sww;
+    f[4] = suu - sww;
+    f[5] = log2(m) / 16.0;
+    f[6] = 1.0 / m;
+    f[7] = (D / rho) * (D / rho);
+}
+
+static void far_rec(far_t *F, int32_t l, int32_t r, double vx, double vy, int mode, double *fx, double *fy) {
+    const int d = split_digit(F, l, r);
+    const double D = ldexp(F->maxmin, -d);
+    double cx, cy, mxx, mxy, myy;
+    cell_summary(F, l, r, &cx, &cy, &mxx, &mxy, &myy);
+    const double rx = cx - vx, ry = cy - vy;
+    const double r2 = rx * rx + ry * ry;
+    if (D * D < F->theta2 * r2) {
+        const double m = (double)(r - l);
+        if (mode == 1 && F->cnt < F->cap) { /* training sample: exact sum over the cell */
+            double ex = 0.0, ey = 0.0;
+            for (int32_t k = l; k < r; k++) {
+                const double dx = F->sx[2 * k] - vx, dy = F->sx[2 * k + 1] - vy;
+                const double q2 = dx * dx + dy * dy;
+                if (q2 > 0) { ex += dx / q2; ey += dy / q2; }
+            }
+            double *f = F->feat + 8 * F->cnt;
+            far_features(m, D, rx, ry, mxx, mxy, myy, f);
+            const double rho = sqrt(r2), ux = rx / rho, uy = ry / rho;
+            const double sc = rho / m;
+            F->targ[2 * F->cnt] = (ex * ux + ey * uy) * sc - 1.0;
+            F->targ[2 * F->cnt + 1] = (-ex * uy + ey * ux) * sc;
+            F->cell[2 * F->cnt] = l;
+            F->cell[2 * F->cnt + 1] = r - l;
+            F->cnt++;
+        }
+        *fx += m * rx / r2;
+        *fy += m * ry / r2;
+        return;
+    }
+    int32_t b[5];
+    b[0] = l;
+    b[4] = r;
+    for (int q = 1; q < 4; q++) b[q] = lower_q(F, l, r, d, q);
+    for (int q = 0; q < 4; q++) {
+        if (b[q + 1] - b[q] >= 2) {
+            far_rec(F, b[q], b[q + 1], vx, vy, mode, fx, fy);
+        } else if (b[q + 1] - b[q] == 1) {
+            const double dx = F->sx[2 * b[q]] - vx, dy = F->sx[2 * b[q] + 1] - vy;
+            const double q2 = dx * dx + dy * dy;
+            if (q2 > 0) { *fx += dx / q2; *fy += dy / q2; }
+        }
+    }
+}
+
+static int far_prepare(int32_t n, const double *xy, uint64_t **key, double **sx, double *maxmin) {
+    frame_t fr = frame_of(n, xy);
+    kv_t *kv = (kv_t *)malloc(sizeof(kv_t) * (size_t)n);
+    *key = (uint64_t *)malloc(8 * (size_t)n);
+    *sx = (double *)malloc(16 * (size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        kv[i].key = morton_key(scale_of(xy[2 * i], fr.minx, fr.maxmin), scale_of(xy[2 * i + 1], fr.miny, fr.maxmin));
+        kv[i].idx = i;
+    }
+    qsort(kv, (size_t)n, sizeof(kv_t), kv_cmp);
+    int dup = 0;
+    for (int32_t k = 0; k < n; k++) {
+        (*key)[k] = kv[k].key;
+        (*sx)[2 * k] = xy[2 * kv[k].idx];
+        (*sx)[2 * k + 1] = xy[2 * kv[k].idx + 1];
+        if (k > 0 && kv[k].key == kv[k - 1].key) dup = 1;
+    }
+    free(kv);
+    *maxmin = fr.maxmin;
+    return dup ? ORC_EDUPKEY : ORC_OK;
+}
+
+int orc_far_forces(int32_t n, const double *xy, double theta, int32_t nq, const double *q, double *out,
+                   int threads) {
+    if (n < 1 || nq < 0) return ORC_EARG;
+    uint64_t *key;
+    double *sx, maxmin;
+    int rc = far_prepare(n, xy, &key, &sx, &maxmin);
+    if (rc == ORC_OK) {
+        if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int32_t a = 0; a < nq; a++) {
+            far_t F = {key, sx, maxmin, theta * theta, 0, 0, NULL, NULL, NULL};
+            double fx = 0.0, fy = 0.0;
+            if (n >= 2) {
+                far_rec(&F, 0, n, q[2 * a], q[2 * a + 1], 0, &fx, &fy);
+            } else {
+                const double dx = sx[0] - q[2 * a], dy = sx[1] - q[2 * a + 1];
+                const double q2 = dx * dx + dy * dy;
+                if (q2 > 0) { fx = dx / q2; fy = dy / q2; }
+            }
+            out[2 * a] = fx;
+            out[2 * a + 1] = fy;
+        }
+    }
+    free(key);
+    free(sx);
+    return rc;
+}
+
+int orc_far_samples(int32_t n, const double *xy, double theta, double qx, double qy, int32_t cap, double *feat,
+                    double *targ, int32_t *cell, int32_t *count, double *force) {
+    if (n < 2 || cap < 0) return ORC_EARG;
+    uint64_t *key;
+    double *sx, maxmin;
+    int rc = far_prepare(n, xy, &key, &sx, &maxmin);
+    if (rc == ORC_OK) {
+        far_t F = {key, sx, maxmin, theta * theta, cap, 0, feat, targ, cell};
+        double fx = 0.0, fy = 0.0;
+        far_rec(&F, 0, n, qx, qy, 1, &fx, &fy);
+        *count = F.cnt;
+        if (force) { force[0] = fx; force[1] = fy; }
+    }
+    free(key);
+    free(sx);
+    return rc;
+}
+
+/* Layout with the physically consistent repulsion: the algo-4 iteration (repulsion from the
+ * iteration-start positions, attraction d * W|d| over every CSR entry with live positions,
+ * sequential batches, STEP *= 0.999) with orc_far_forces in place of calcRepForce. */
+int orc_far_layout(int32_t n, const int32_t *rowptr, const int32_t *colids, const float *w_edge, double *xy,
+                   int iters, int batch, double theta, double weight, double step0, int threads, double *energy) {
+    if (n <= 0 || batch <= 0) return ORC_EARG;
+    if (threads > 0) omp_set_num_threads(threads);
+    double *rep = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *prev = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double *start = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    double STEP = step0, E = 0.0;
+    const int nb = (int)ceil(1.0 * n / batch);
+    int rc = ORC_OK;
+    for (int loop = 0; loop < iters && rc == ORC_OK; loop++) {
+        E = 0.0;
+        memcpy(start, xy, sizeof(double) * 2 * (size_t)n);
+        rc = orc_far_forces(n, start, theta, n, start, rep, threads);
+        if (rc != ORC_OK) break;
+        for (int b = 0; b < nb; b++) {
+            const int64_t i0 = (int64_t)b * batch, i1 = (int64_t)(b + 1) * batch;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = i0; i < i1; i++) {
+                if (i >= n) continue;
+                double fx = 0.0, fy = 0.0;
+                const double vx = xy[2 * i], vy = xy[2 * i + 1];
+                for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+                    const int32_t j = colids[k];
+                    const double dx = xy[2 * j] - vx, dy = xy[2 * j + 1] - vy;
+                    const double W = weight * (w_edge ? (double)w_edge[k] : 1.0);
+                    const double s = W * sqrt(dx * dx + dy * dy);
+                    fx = fx + dx * s;
+                    fy = fy + dy * s;
+                }
+                prev[2 * i] = fx - rep[2 * i];
+                prev[2 * i + 1] = fy - rep[2 * i + 1];
+            }
+            for (int64_t i = i0; i < i1 && i < n; i++) {
+                const double fx = prev[2 * i], fy = prev[2 * i + 1];
+                const double mag = sqrt(fx * fx + fy * fy);
+                xy[2 * i] = xy[2 * i] + (fx / mag) * STEP;
+                xy[2 * i + 1] = xy[2 * i + 1] + (fy / mag) * STEP;
+                E += fx * fx + fy * fy;
+            }
+        }
+        STEP = STEP * 0.999;
+    }
+    free(rep);
+    free(prev);
+    free(start);
+    if (energy) *energy = E;
+    return rc;
+}

@@ -76,6 +76,27 @@ int orc_bh_forces(int32_t n, const double *xy, double theta, int32_t nq, const d
 int orc_exact_repulsion(int32_t n, const double *xy, int32_t nq, const int32_t *targets, double *out,
                         int threads);
 
+
+/* Physically consistent Barnes-Hut (no reference counterpart; the baseline for the learned
+ * far field): for each query v, sum over the compressed quadtree of the Morton-sorted points
+ * (cells where the points first split, side MAXMIN * 2^-d) of m * (c - v)/|c - v|^2 for every
+ * cell with side^2 < theta^2 |c - v|^2 (c = mean of the cell's points, m = count), else
+ * recursing; single points add (p - v)/|p - v|^2 when |p - v| > 0.  theta = 0 gives the
+ * exact sum. */
+int orc_far_forces(int32_t n, const double *xy, double theta, int32_t nq, const double *q, double *out,
+                   int threads);
+
+/* The cells accepted for one query (qx, qy) by the traversal above, in depth-first quadrant
+ * order: feat[8k..8k+8) the learned far-field features, targ[2k..2k+2) the exact-sum targets
+ * (a, b) with F_cell = (m/rho)((1 + a) u + b w), cell[2k] = first Morton rank, cell[2k+1] = m.
+ * *force = the monopole force of the whole traversal. */
+int orc_far_samples(int32_t n, const double *xy, double theta, double qx, double qy, int32_t cap, double *feat,
+                    double *targ, int32_t *cell, int32_t *count, double *force);
+
+/* The algo-4 iteration with orc_far_forces as the repulsion (theta = 0: exact repulsion). */
+int orc_far_layout(int32_t n, const int32_t *rowptr, const int32_t *colids, const float *w_edge, double *xy,
+                   int iters, int batch, double theta, double weight, double step0, int threads, double *energy);
+
 #ifdef __cplusplus
 }
 #endif

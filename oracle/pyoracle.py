@@ -48,6 +48,11 @@ def lib():
                                   _f64p, _f64p, _f64p, _i32p]
         L.orc_bh_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, _f64p, _f64p, C.c_int]
         L.orc_exact_repulsion.argtypes = [C.c_int32, _f64p, C.c_int32, _i32p, _f64p, C.c_int]
+        L.orc_far_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, _f64p, _f64p, C.c_int]
+        L.orc_far_samples.argtypes = [C.c_int32, _f64p, C.c_double, C.c_double, C.c_double, C.c_int32, _f64p,
+                                      _f64p, _i32p, C.POINTER(C.c_int32), _f64p]
+        L.orc_far_layout.argtypes = [C.c_int32, _i32p, _i32p, C.c_void_p, _f64p, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -154,3 +159,37 @@ def exact_repulsion(xy, targets, threads=0):
     out = np.zeros((len(t), 2))
     _chk(lib().orc_exact_repulsion(len(xy), xy, len(t), t, out, threads), "exact_repulsion")
     return out
+
+
+def far_forces(xy, theta, queries=None, threads=0):
+    """Physically consistent BH (mass-weighted monopole, d/|d|^2 law); theta = 0 is exact."""
+    xy = np.ascontiguousarray(xy, np.float64)
+    q = xy if queries is None else np.ascontiguousarray(queries, np.float64)
+    out = np.zeros((len(q), 2))
+    _chk(lib().orc_far_forces(len(xy), xy, theta, len(q), q, out, threads), "far_forces")
+    return out
+
+
+def far_samples(xy, theta, query, cap=100000):
+    """Accepted cells of one query: (features[k, 8], targets[k, 2], cells[k, 2], monopole force)."""
+    xy = np.ascontiguousarray(xy, np.float64)
+    feat = np.zeros((cap, 8))
+    targ = np.zeros((cap, 2))
+    cell = np.zeros((cap, 2), np.int32)
+    cnt = C.c_int32(0)
+    force = np.zeros(2)
+    _chk(lib().orc_far_samples(len(xy), xy, theta, float(query[0]), float(query[1]), cap, feat, targ, cell,
+                               C.byref(cnt), force), "far_samples")
+    k = cnt.value
+    return feat[:k], targ[:k], cell[:k], force
+
+
+def far_layout(rowptr, colids, xy, iters, batch, theta, weight=1.0, step0=1.0, w_edge=None, threads=0):
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colids = np.ascontiguousarray(colids, np.int32)
+    xy = np.array(xy, np.float64, copy=True, order="C")
+    w = None if w_edge is None else np.ascontiguousarray(w_edge, np.float32)
+    e = C.c_double(0)
+    _chk(lib().orc_far_layout(len(rowptr) - 1, rowptr, colids, None if w is None else w.ctypes.data, xy, iters,
+                              batch, theta, weight, step0, threads, C.byref(e)), "far_layout")
+    return xy, e.value
