@@ -19,8 +19,9 @@ STATUS = {0: "BL_OK", -1: "BL_E_ARG", -2: "BL_E_IO", -3: "BL_E_FORMAT", -4: "BL_
           -5: "BL_E_DUPKEY", -6: "BL_E_NOMEM", -7: "BL_E_GRAPH"}
 ALGO_CACHE, ALGO_LINLOG, ALGO_BH, ALGO_BHCACHE, ALGO_FA = 2, 3, 4, 6, 8
 # physically consistent Barnes-Hut layouts (no reference counterpart)
-ALGO_FAR_MONO, ALGO_FAR_EXACT = 40, 41
-FAR_MONO, FAR_EXACT = 1, 2
+ALGO_FAR_MONO, ALGO_FAR_EXACT, ALGO_FAR_LEARNED, ALGO_FAR_QUAD = 40, 41, 42, 43
+FAR_MONO, FAR_EXACT, FAR_LEARNED, FAR_QUAD = 1, 2, 3, 4
+FAR_MODEL_PARAMS, FAR_FEATURES = 9026, 8
 PREC_MIXED, PREC_FP64, PREC_FP32 = 0, 1, 2
 BATCHED_MAX_N = 8192
 
@@ -48,6 +49,7 @@ _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 
 def lib():
@@ -73,7 +75,14 @@ def lib():
         L.bl_bh_tree.argtypes = [C.c_int32, _f64p, _i32p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  _f64p, _f64p, _f64p, _i32p]
         L.bl_bh_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, _f64p, _f64p, C.c_int32]
-        L.bl_far_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, C.c_int32, _f64p, _f64p]
+        L.bl_far_forces.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, C.c_void_p, C.c_int32, _f64p, _f64p]
+        L.bl_far_samples.argtypes = [C.c_int32, _f64p, C.c_double, C.c_int32, _f64p, C.c_int32, _f32p, _f32p, _i32p]
+        L.bl_far_model_init.argtypes = [_f32p, C.c_uint64]
+        L.bl_far_model_eval.argtypes = [_f32p, C.c_int32, _f32p, _f32p]
+        L.bl_far_model_grad.argtypes = [_f32p, C.c_int32, _f32p, _f32p, C.POINTER(C.c_float), _f32p]
+        L.bl_far_model_train.argtypes = [_f32p, C.c_int32, _f32p, _f32p, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                         _f32p]
+        L.bl_graph_set_far_model.argtypes = [C.c_void_p, C.c_void_p]
         L.bl_run_batched.argtypes = [C.c_int32, _i32p, _i64p, _i32p, _i32p, _f64p, C.POINTER(bl_params), _f64p,
                                      _f64p, C.POINTER(bl_stats)]
         L.bl_device_info.argtypes = [C.c_char_p, C.c_int32]
@@ -156,6 +165,11 @@ class Graph:
         _chk(lib().bl_run(self._h, out, C.byref(p), C.byref(st)), "bl_run")
         return out, _stats(st)
 
+    def set_far_model(self, params):
+        """Attaches the learned far-field model used by ALGO_FAR_LEARNED (None detaches)."""
+        m = None if params is None else np.ascontiguousarray(params, np.float32)
+        _chk(lib().bl_graph_set_far_model(self._h, None if m is None else m.ctypes.data), "bl_graph_set_far_model")
+
     def run_device(self, d_xy_ptr: int, stream: int | None = None, **kw) -> Stats:
         """Layout on device-resident positions (a raw device pointer, e.g. tensor.data_ptr())."""
         p = params(**kw)
@@ -220,13 +234,61 @@ def bh_forces(xy, theta, queries=None, precision=PREC_FP32):
     return out
 
 
-def far_forces(xy, theta, mode=FAR_MONO, queries=None):
-    """Repulsion sum_j (x_j - q)/|x_j - q|^2 by mass-weighted Barnes-Hut (FAR_MONO) or exactly."""
+def far_forces(xy, theta, mode=FAR_MONO, queries=None, model=None):
+    """Repulsion sum_j (x_j - q)/|x_j - q|^2: Barnes-Hut with monopoles (FAR_MONO), second-order
+    multipoles (FAR_QUAD), the learned correction (FAR_LEARNED, needs model) or exactly."""
     xy = np.ascontiguousarray(xy, dtype=np.float64)
     q = xy if queries is None else np.ascontiguousarray(queries, dtype=np.float64)
     out = np.zeros((len(q), 2))
-    _chk(lib().bl_far_forces(len(xy), xy, theta, mode, len(q), q, out), "bl_far_forces")
+    m = None if model is None else np.ascontiguousarray(model, dtype=np.float32)
+    _chk(lib().bl_far_forces(len(xy), xy, theta, mode, None if m is None else m.ctypes.data, len(q), q, out),
+         "bl_far_forces")
     return out
+
+
+def far_samples(xy, theta, queries, cap=512):
+    """Training rows of the accepted cells of each query: (features[k, 8], targets[k, 2]) with
+    the exact (a, b), concatenated in query order."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    q = np.ascontiguousarray(queries, dtype=np.float64)
+    feat = np.zeros((len(q) * cap, FAR_FEATURES), np.float32)
+    targ = np.zeros((len(q) * cap, 2), np.float32)
+    cnt = np.zeros(len(q), np.int32)
+    _chk(lib().bl_far_samples(len(xy), xy, theta, len(q), q, cap, feat, targ, cnt), "bl_far_samples")
+    keep = (np.arange(cap)[None, :] < cnt[:, None]).ravel()
+    return feat[keep], targ[keep]
+
+
+def far_model_init(seed=1):
+    p = np.zeros(FAR_MODEL_PARAMS, np.float32)
+    _chk(lib().bl_far_model_init(p, seed), "bl_far_model_init")
+    return p
+
+
+def far_model_eval(params, feat):
+    feat = np.ascontiguousarray(feat, np.float32)
+    out = np.zeros((len(feat), 2), np.float32)
+    _chk(lib().bl_far_model_eval(np.ascontiguousarray(params, np.float32), len(feat), feat, out), "bl_far_model_eval")
+    return out
+
+
+def far_model_grad(params, feat, targ):
+    feat = np.ascontiguousarray(feat, np.float32)
+    targ = np.ascontiguousarray(targ, np.float32)
+    g = np.zeros(FAR_MODEL_PARAMS, np.float32)
+    loss = C.c_float(0)
+    _chk(lib().bl_far_model_grad(np.ascontiguousarray(params, np.float32), len(feat), feat, targ, C.byref(loss), g),
+         "bl_far_model_grad")
+    return loss.value, g
+
+
+def far_model_train(params, feat, targ, epochs, batch=4096, lr=1e-3, seed=1):
+    p = np.array(params, np.float32, copy=True)
+    feat = np.ascontiguousarray(feat, np.float32)
+    targ = np.ascontiguousarray(targ, np.float32)
+    losses = np.zeros(max(epochs, 1), np.float32)
+    _chk(lib().bl_far_model_train(p, len(feat), feat, targ, epochs, batch, lr, seed, losses), "bl_far_model_train")
+    return p, losses[:epochs]
 
 
 def run_batched(graphs, xys, **kw):

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "bh_common.h"
+#include "mlp.h"
 
 namespace bl {
 
@@ -150,7 +151,7 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
     if (!g) return BL_E_ARG;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
-                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch};
+                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model};
     for (DevBuf *b : all) b->release();
     g->bh.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -172,7 +173,7 @@ static int validate_params(const bl_params *p) {
         return BL_E_ARG;
     switch (p->algo) {
         case BL_ALGO_CACHE: case BL_ALGO_LINLOG: case BL_ALGO_BH: case BL_ALGO_BHCACHE: case BL_ALGO_FA:
-        case BL_ALGO_FAR_MONO: case BL_ALGO_FAR_EXACT:
+        case BL_ALGO_FAR_MONO: case BL_ALGO_FAR_EXACT: case BL_ALGO_FAR_LEARNED: case BL_ALGO_FAR_QUAD:
             return BL_OK;
         default:
             return BL_E_ARG;
@@ -215,6 +216,12 @@ static int run_on_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats
             break;
         case BL_ALGO_FAR_EXACT:
             rc = run_bh(g, d_xy, p, p->step0, p->iters, 0.0, &local, FAR_EXACT);
+            break;
+        case BL_ALGO_FAR_QUAD:
+            rc = run_bh(g, d_xy, p, p->step0, p->iters, p->bht, &local, FAR_QUAD);
+            break;
+        case BL_ALGO_FAR_LEARNED:
+            rc = g->has_far_model ? run_bh(g, d_xy, p, p->step0, p->iters, p->bht, &local, FAR_LEARNED) : BL_E_ARG;
             break;
         default:
             rc = BL_E_ARG;
@@ -287,13 +294,93 @@ extern "C" int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t n
     return bh_forces_host(n, xy, theta, nq, q, out, precision);
 }
 
-extern "C" int bl_far_forces(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
-                             double *out) {
+static int far_mode_of(int32_t mode) {
+    switch (mode) {
+        case BL_FAR_MONO: return FAR_MONO;
+        case BL_FAR_EXACT: return FAR_EXACT;
+        case BL_FAR_QUAD: return FAR_QUAD;
+        case BL_FAR_LEARNED: return FAR_LEARNED;
+        default: return 0;
+    }
+}
+
+extern "C" int bl_far_forces(int32_t n, const double *xy, double theta, int32_t mode, const float *model, int32_t nq,
+                             const double *q, double *out) {
     if (n <= 0 || !xy || nq < 0 || (nq > 0 && (!q || !out)) || theta < 0.0) return BL_E_ARG;
-    if (mode != BL_FAR_MONO && mode != BL_FAR_EXACT) return BL_E_ARG;
+    const int fm = far_mode_of(mode);
+    if (!fm || (fm == FAR_LEARNED && !model)) return BL_E_ARG;
     int rc = check_device_once();
     if (rc) return rc;
-    return far_forces_host(n, xy, theta, mode == BL_FAR_MONO ? FAR_MONO : FAR_EXACT, nq, q, out);
+    return far_forces_host(n, xy, theta, fm, model, nq, q, out);
+}
+
+extern "C" int bl_far_samples(int32_t n, const double *xy, double theta, int32_t nq, const double *q, int32_t cap,
+                              float *feat, float *targ, int32_t *count) {
+    if (n < 2 || !xy || nq < 0 || cap < 1 || theta <= 0.0 || (nq > 0 && (!q || !feat || !targ || !count)))
+        return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return far_samples_host(n, xy, theta, nq, q, cap, feat, targ, count);
+}
+
+// Glorot-uniform weights from xorshift64* (seeded), zero biases; the output layer scaled by
+// 0.1 so that an untrained model starts near the plain monopole.
+extern "C" int bl_far_model_init(float *params, uint64_t seed) {
+    if (!params) return BL_E_ARG;
+    uint64_t st = seed ? seed : 0x9E3779B97F4A7C15ull;
+    auto unif = [&]() {
+        st ^= st >> 12;
+        st ^= st << 25;
+        st ^= st >> 27;
+        return (double)((st * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    auto fill = [&](int off, int rows, int cols, double gain) {
+        const double a = gain * sqrt(6.0 / (double)(rows + cols));
+        for (int k = 0; k < rows * cols; ++k) params[off + k] = (float)((2.0 * unif() - 1.0) * a);
+    };
+    memset(params, 0, sizeof(float) * BL_FAR_MODEL_PARAMS);
+    fill(MLP_W1, MLP_H, MLP_IN, 1.0);
+    fill(MLP_W2, MLP_H, MLP_H, 1.0);
+    fill(MLP_W3, MLP_H, MLP_H, 1.0);
+    fill(MLP_W4, MLP_OUT, MLP_H, 0.1);
+    return BL_OK;
+}
+
+extern "C" int bl_far_model_eval(const float *params, int32_t n, const float *feat, float *out) {
+    if (!params || n < 0 || (n > 0 && (!feat || !out))) return BL_E_ARG;
+    if (n == 0) return BL_OK;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return far_model_eval_host(params, n, feat, out);
+}
+
+extern "C" int bl_far_model_grad(const float *params, int32_t n, const float *feat, const float *targ, float *loss,
+                                 float *grad) {
+    if (!params || n <= 0 || !feat || !targ) return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return far_model_grad_host(params, n, feat, targ, loss, grad);
+}
+
+extern "C" int bl_far_model_train(float *params, int32_t n, const float *feat, const float *targ, int32_t epochs,
+                                  int32_t batch, float lr, uint64_t seed, float *losses) {
+    if (!params || n <= 0 || !feat || !targ || epochs < 0 || batch <= 0 || !(lr > 0.0f)) return BL_E_ARG;
+    int rc = check_device_once();
+    if (rc) return rc;
+    return far_model_train_host(params, n, feat, targ, epochs, batch, lr, seed, losses);
+}
+
+extern "C" int bl_graph_set_far_model(bl_graph *g, const float *params) {
+    if (!g) return BL_E_ARG;
+    if (!params) {
+        g->has_far_model = false;
+        return BL_OK;
+    }
+    if (g->far_model.ensure(sizeof(float) * BL_FAR_MODEL_PARAMS)) return BL_E_NOMEM;
+    if (hipMemcpy(g->far_model.p, params, sizeof(float) * BL_FAR_MODEL_PARAMS, hipMemcpyHostToDevice) != hipSuccess)
+        return BL_E_DEVICE;
+    g->has_far_model = true;
+    return BL_OK;
 }
 
 extern "C" int bl_run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr,

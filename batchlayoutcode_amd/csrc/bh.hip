@@ -950,7 +950,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
             if (rc) return rc;
         }
         prof.begin(0);
-        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, far_mode, nullptr, s)
+        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, far_mode, g->has_far_model ? g->far_model.as<float>() : nullptr, s)
                       : launch_rep(g->bh, n, theta, p->precision, s);
         prof.end();
         if (rc) return rc;

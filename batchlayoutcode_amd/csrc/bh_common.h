@@ -26,9 +26,10 @@ constexpr int ST_DUPKEY = 1;
 int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 
 // Repulsion of every vertex for the physically consistent layouts (farfield.hip): into
-// w.rep (by vertex id).  FAR_MONO and FAR_LEARNED read the tree of d_pos already built in w
-// by bh_build_tree; FAR_EXACT sums over all points.
-enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3 };
+// w.rep (by vertex id).  FAR_EXACT sums over all points; the other modes read the tree of
+// d_pos already built in w by bh_build_tree.  FAR_LEARNED needs model (device, MLP_NPARAM
+// floats).
+enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3, FAR_QUAD = 4 };
 int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode, const float *model,
                   hipStream_t s);
 

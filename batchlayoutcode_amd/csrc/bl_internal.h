@@ -64,6 +64,8 @@ struct bl_graph {
     // workspace
     bl::DevBuf pos, part, eterm, energy, scratch;
     bl::BHWork bh;
+    bl::DevBuf far_model;  // learned far-field parameters (bl_graph_set_far_model)
+    bool has_far_model = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -164,8 +166,16 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
 int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta,
            bl_stats *st, int far_mode = 0);
 // farfield.hip
-int far_forces_host(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
-                    double *out);
+int far_forces_host(int32_t n, const double *xy, double theta, int32_t mode, const float *model, int32_t nq,
+                    const double *q, double *out);
+int far_samples_host(int32_t n, const double *xy, double theta, int32_t nq, const double *q, int32_t cap,
+                     float *feat, float *targ, int32_t *count);
+// mlp.hip
+int far_model_grad_host(const float *params, int32_t n, const float *feat, const float *targ, float *loss,
+                        float *grad);
+int far_model_eval_host(const float *params, int32_t n, const float *feat, float *out);
+int far_model_train_host(float *params, int32_t n, const float *feat, const float *targ, int32_t epochs,
+                         int32_t batch, float lr, uint64_t seed, float *losses);
 int bh_morton_host(int32_t n, const double *xy, uint32_t *xs, uint32_t *ys, double *maxmin);
 int bh_tree_host(int32_t n, const double *xy, int32_t *perm, int32_t max_nodes, int32_t *nnodes,
                  int32_t *root, double *diam, double *cx, double *cy, int32_t *child);

@@ -4,19 +4,28 @@
 // whatever the cell holds.  The layouts here use the repulsion law of algo 2
 // (algorithms.cpp:764-768: a pair contributes d/|d|^2) for every vertex pair and approximate
 // the far field of a cell by its mass-weighted monopole m (c - v)/|c - v|^2 (c = mean of the
-// cell's points, m = their count), or by the monopole corrected by a learned model of the
-// cell's shape.  The exact sum over all points is the ground truth of both.
+// cell's points, m = their count), optionally corrected for the cell's shape: in the frame of
+// u = (c - v)/|c - v| and w = u turned by +90 degrees, the cell's exact field is
+//     F = (m/rho) ((1 + a) u + b w),     rho = |c - v|,
+// and the correction (a, b) is either the second-order multipole term (a = S_uu - S_ww,
+// b = -2 S_uw with S the cell's normalised second moments, from 1/conj(w + delta) expanded
+// in delta) or the output of the learned model of mlp.h scaled by (side/rho)^2.
 //
 //   k_far_moments  count, mean, central second moments and first Morton rank of every cell,
 //                  bottom-up over the tree of bh.hip (last child to arrive computes the parent)
 //   k_far_trav     traversal, one wave per 64 Morton-consecutive queries with a shared stack
-//                  of (node, lane mask); a cell is accepted when side^2 < theta^2 |c - v|^2
+//                  of (node, lane mask); a cell is accepted when side^2 < theta^2 |c - v|^2.
+//                  The learned mode queues accepted cells per wave in LDS and evaluates the
+//                  model on full tiles of 64 cells with MFMA
+//   k_far_samples  training data: features and exact (a, b) of every accepted cell
 //   k_far_exact    exact sum over all points for every query
 #include <math.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "bh_common.h"
+#include "mlp.h"
 #include "pair_math.h"
 
 namespace bl {
@@ -24,6 +33,7 @@ namespace {
 
 constexpr int WG = 256;
 constexpr int STACK = 128;
+constexpr int QCAP = 128;  // queued cells per wave (a step adds at most 64)
 
 struct FarCell {
     double cx, cy;          // mean of the cell's points
@@ -101,17 +111,88 @@ __global__ __launch_bounds__(WG) void k_far_moments(const BHNode *__restrict__ n
     }
 }
 
-// Repulsion F(v) = sum over accepted cells of m (c - v)/|c - v|^2 + sum over the single points
-// met of (p - v)/|p - v|^2.  The opening test and the terms are FP64-grade (FP64 differences,
+// Features of an accepted cell seen from v; (dx, dy) = c - v, r2 = |c - v|^2, D = side:
+// [D/rho, S_uu, S_uw, S_ww, S_uu - S_ww, log2(m)/16, 1/m, (D/rho)^2], S = M2 / (m rho^2) in the
+// (u, w) frame.
+__device__ __forceinline__ void far_features(const FarCell &cl, double dx, double dy, double r2, double D, float *f) {
+    const double rho = sqrt(r2);
+    const double ux = dx / rho, uy = dy / rho, wx = -uy, wy = ux;
+    const double m = (double)cl.m;
+    const double s = 1.0 / (m * r2);
+    const double suu = (ux * ux * cl.mxx + 2.0 * ux * uy * cl.mxy + uy * uy * cl.myy) * s;
+    const double suw = (ux * wx * cl.mxx + (ux * wy + uy * wx) * cl.mxy + uy * wy * cl.myy) * s;
+    const double sww = (wx * wx * cl.mxx + 2.0 * wx * wy * cl.mxy + wy * wy * cl.myy) * s;
+    const double t = D / rho;
+    f[0] = (float)t;
+    f[1] = (float)suu;
+    f[2] = (float)suw;
+    f[3] = (float)sww;
+    f[4] = (float)(suu - sww);
+    f[5] = (float)(log2(m) / 16.0);
+    f[6] = (float)(1.0 / m);
+    f[7] = (float)(t * t);
+}
+
+enum TravMode : int { TRAV_MONO = 0, TRAV_QUAD = 1, TRAV_LEARNED = 2 };
+constexpr size_t QOFF = (sizeof(MlpLds) + 15) & ~(size_t)15;
+
+// Per-wave queue of accepted cells waiting for the model (learned mode).
+struct FarQueue {
+    float feat[QCAP][MLP_IN];
+    float mono[QCAP][2];  // the cell's monopole m (c - v)/|c - v|^2
+    int owner[QCAP];      // lane of the query
+    float corr[64][2];    // model correction of the evaluated tile
+};
+constexpr size_t LEARNED_LDS = QOFF + sizeof(FarQueue) * (WG / 64);
+
+// Evaluates the first 64 queued cells and adds (f7 y0) mono + (f7 y1) mono_perp to their
+// owners, each lane summing its own cells in queue order.
+__device__ void far_flush(const MlpLds &L, FarQueue &Q, int count, int lane, double &fx, double &fy) {
+    auto feat = [&](int s, int f) { return s < count ? Q.feat[s][f] : 0.0f; };
+    f32x4 Y[4];
+    mlp_forward(L, feat, Y, lane);
+    if (lane < 16) {
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+            const int s = sb * 16 + lane;
+            const float t2 = Q.feat[s][7];
+            const float a = t2 * Y[sb][0], b = t2 * Y[sb][1];
+            Q.corr[s][0] = a * Q.mono[s][0] - b * Q.mono[s][1];
+            Q.corr[s][1] = a * Q.mono[s][1] + b * Q.mono[s][0];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int lim = min(count, 64);
+    for (int s = 0; s < lim; ++s) {
+        if (Q.owner[s] == lane) {
+            fx += (double)Q.corr[s][0];
+            fy += (double)Q.corr[s][1];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Repulsion F(v) = sum over accepted cells of their far field + sum over the single points met
+// of (p - v)/|p - v|^2.  Opening test and monopole terms are FP64-grade (FP64 differences,
 // FP32-seeded reciprocal with an FP64 Newton step); sums in FP64.
+template <int MODE>
 __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpos, const int *__restrict__ qid,
                                                  int nq, const double2 *__restrict__ spos,
                                                  const BHNode *__restrict__ nodes, const FarCell *__restrict__ cells,
                                                  const int *__restrict__ misc, const double *__restrict__ frame,
-                                                 double theta, double2 *__restrict__ out) {
+                                                 double theta, const float *__restrict__ model,
+                                                 double2 *__restrict__ out) {
     __shared__ int stk_node[WG / 64][STACK];
     __shared__ uint64_t stk_mask[WG / 64][STACK];
+    // learned mode: the model and the per-wave queues in dynamic LDS (LEARNED_LDS bytes)
+    extern __shared__ __align__(16) unsigned char dyn[];
+    MlpLds *L = reinterpret_cast<MlpLds *>(dyn);
+    FarQueue *Qs = reinterpret_cast<FarQueue *>(dyn + QOFF);
     if (misc[0]) return;
+    if constexpr (MODE == TRAV_LEARNED) {
+        mlp_load(*L, model, threadIdx.x, WG);
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int k = (blockIdx.x * (WG / 64) + w) * 64 + lane;
     const bool valid = k < nq;
@@ -129,6 +210,8 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         }
         return;
     }
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int qn = 0;  // queued cells of this wave (learned mode)
     if (lane == 0) {
         stk_node[w][0] = root;
         stk_mask[w][0] = m0;
@@ -149,8 +232,47 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         const bool acc = D * D < th2 * r2;
         if (act && acc) {
             const double s = (double)cl.m * inv_r2<true>(dx, dy);
-            fx = fma(dx, s, fx);
-            fy = fma(dy, s, fy);
+            const double mx = dx * s, my = dy * s;
+            fx += mx;
+            fy += my;
+            if constexpr (MODE == TRAV_QUAD) {
+                float f[8];
+                far_features(cl, dx, dy, r2, D, f);
+                const double a = (double)f[4], b = -2.0 * (double)f[2];
+                fx += a * mx - b * my;
+                fy += a * my + b * mx;
+            }
+        }
+        if constexpr (MODE == TRAV_LEARNED) {
+            const uint64_t accm = __ballot(act && acc);
+            if (accm) {
+                if (act && acc) {
+                    const int slot = qn + __popcll(accm & lt);
+                    FarQueue &Q = Qs[w];
+                    far_features(cl, dx, dy, r2, D, Q.feat[slot]);
+                    const double s = (double)cl.m * inv_r2<true>(dx, dy);
+                    Q.mono[slot][0] = (float)(dx * s);
+                    Q.mono[slot][1] = (float)(dy * s);
+                    Q.owner[slot] = lane;
+                }
+                qn += __popcll(accm);
+                __builtin_amdgcn_wave_barrier();
+                if (qn >= 64) {
+                    FarQueue &Q = Qs[w];
+                    far_flush(*L, Q, 64, lane, fx, fy);
+                    // move the remaining qn - 64 (< 64) entries to the front
+                    const int rest = qn - 64;
+                    if (lane < rest) {
+#pragma unroll
+                        for (int f = 0; f < MLP_IN; ++f) Q.feat[lane][f] = Q.feat[64 + lane][f];
+                        Q.mono[lane][0] = Q.mono[64 + lane][0];
+                        Q.mono[lane][1] = Q.mono[64 + lane][1];
+                        Q.owner[lane] = Q.owner[64 + lane];
+                    }
+                    qn = rest;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
         }
         const uint64_t rej = __ballot(act && !acc);
         if (rej) {
@@ -176,7 +298,93 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
             __builtin_amdgcn_wave_barrier();
         }
     }
+    if constexpr (MODE == TRAV_LEARNED) {
+        if (qn > 0) far_flush(*L, Qs[w], qn, lane, fx, fy);
+    }
     if (valid) out[qid[k]] = make_double2(fx, fy);
+}
+
+// Training samples: for query k, its accepted cells (at most cap) with their features and
+// the exact (a, b) of F_cell = (m/rho)((1 + a) u + b w), the exact field summed over the cell's
+// points in FP64.  Rows k*cap .. k*cap + count[k] - 1, in traversal order.
+__global__ __launch_bounds__(WG) void k_far_samples(const double2 *__restrict__ qpos, int nq,
+                                                    const double2 *__restrict__ spos,
+                                                    const BHNode *__restrict__ nodes,
+                                                    const FarCell *__restrict__ cells, const int *__restrict__ misc,
+                                                    const double *__restrict__ frame, double theta, int cap,
+                                                    float *__restrict__ feat, float *__restrict__ targ,
+                                                    int *__restrict__ count) {
+    __shared__ int stk_node[WG / 64][STACK];
+    __shared__ uint64_t stk_mask[WG / 64][STACK];
+    if (misc[0]) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = (blockIdx.x * (WG / 64) + w) * 64 + lane;
+    const bool valid = k < nq;
+    const double2 p = qpos[valid ? k : nq - 1];
+    const double maxmin = frame[2], th2 = theta * theta;
+    const int root = misc[1];
+    const uint64_t m0 = __ballot(valid);
+    if (m0 == 0) return;
+    int cnt = 0;
+    if (root >= 0) {
+        if (lane == 0) {
+            stk_node[w][0] = root;
+            stk_mask[w][0] = m0;
+        }
+        int sp = 1;
+        __builtin_amdgcn_wave_barrier();
+        while (sp > 0) {
+            --sp;
+            const int v = uni(stk_node[w][sp]);
+            const uint64_t m = stk_mask[w][sp];
+            __builtin_amdgcn_wave_barrier();
+            const bool act = (m >> lane) & 1ull;
+            const FarCell cl = cells[v];
+            const BHNode nd = nodes[v];
+            const double dx = cl.cx - p.x, dy = cl.cy - p.y;
+            const double r2 = dx * dx + dy * dy;
+            const double D = ldexp(maxmin, -nd.depth);
+            const bool acc = D * D < th2 * r2;
+            const bool take = act && acc && cnt < cap;
+            if (__ballot(take)) {
+                double ex = 0.0, ey = 0.0;
+                const int lo = uni(cl.lo), hi = lo + uni(cl.m);
+                for (int j = lo; j < hi; ++j) {  // same points for every lane: broadcast loads
+                    const double2 q = spos[j];
+                    const double qx = q.x - p.x, qy = q.y - p.y;
+                    const double q2 = qx * qx + qy * qy;
+                    if (take && q2 > 0.0) {
+                        ex += qx / q2;
+                        ey += qy / q2;
+                    }
+                }
+                if (take) {
+                    const size_t row = (size_t)k * cap + cnt;
+                    far_features(cl, dx, dy, r2, D, feat + row * MLP_IN);
+                    const double rho = sqrt(r2), ux = dx / rho, uy = dy / rho, sc = rho / (double)cl.m;
+                    targ[2 * row] = (float)((ex * ux + ey * uy) * sc - 1.0);
+                    targ[2 * row + 1] = (float)((-ex * uy + ey * ux) * sc);
+                    ++cnt;
+                }
+            }
+            const uint64_t rej = __ballot(act && !acc);
+            if (rej) {
+                const int c[4] = {nd.ch.x, nd.ch.y, nd.ch.z, nd.ch.w};
+#pragma unroll
+                for (int q = 3; q >= 0; --q) {
+                    if (c[q] >= 0) {
+                        if (lane == 0) {
+                            stk_node[w][sp] = c[q];
+                            stk_mask[w][sp] = rej;
+                        }
+                        ++sp;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (valid) count[k] = cnt;
 }
 
 // Exact F(v) = sum_j (x_j - v)/|x_j - v|^2 over all points (the self pair and coincident
@@ -205,12 +413,65 @@ int ensure_far(BHWork &w, int n) {
     return BL_OK;
 }
 
+// cell summaries of the tree already built in w
+int far_cells_of(BHWork &w, int n, hipStream_t s) {
+    int rc = ensure_far(w, n);
+    if (rc) return rc;
+    if (n >= 2) {
+        BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
+        hipLaunchKernelGGL(k_far_moments, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
+                           w.spos.as<double2>(), w.misc.as<int>(), w.far_cells.as<FarCell>(), w.far_arrive.as<int>());
+        BL_HIP(hipGetLastError());
+    }
+    return BL_OK;
+}
+
+int launch_trav(BHWork &w, const double2 *qpos, const int *qid, int nq, double theta, int mode, const float *model,
+                double2 *out, hipStream_t s) {
+    const int g64 = ((nq + 63) / 64 + WG / 64 - 1) / (WG / 64);
+    const BHNode *nodes = w.node_child.as<BHNode>();
+    const FarCell *cells = w.far_cells.as<FarCell>();
+    const int *misc = w.misc.as<int>();
+    const double2 *spos = w.spos.as<double2>();
+    const double *frame = w.frame.as<double>();
+    if (mode == FAR_MONO)
+        hipLaunchKernelGGL(k_far_trav<TRAV_MONO>, dim3(g64), dim3(WG), 0, s, qpos, qid, nq, spos, nodes, cells, misc,
+                           frame, theta, model, out);
+    else if (mode == FAR_QUAD)
+        hipLaunchKernelGGL(k_far_trav<TRAV_QUAD>, dim3(g64), dim3(WG), 0, s, qpos, qid, nq, spos, nodes, cells, misc,
+                           frame, theta, model, out);
+    else if (mode == FAR_LEARNED && model)
+        hipLaunchKernelGGL(k_far_trav<TRAV_LEARNED>, dim3(g64), dim3(WG), LEARNED_LDS, s, qpos, qid, nq, spos, nodes, cells,
+                           misc, frame, theta, model, out);
+    else
+        return BL_E_ARG;
+    BL_HIP(hipGetLastError());
+    return BL_OK;
+}
+
+struct HostRun {
+    hipStream_t s = nullptr;
+    BHWork w;
+    DevBuf pts, qs, qi, res, mdl, f, t, c;
+    ~HostRun() {
+        if (s) (void)hipStreamSynchronize(s);
+        w.release();
+        DevBuf *all[] = {&pts, &qs, &qi, &res, &mdl, &f, &t, &c};
+        for (DevBuf *b : all) b->release();
+        if (s) (void)hipStreamDestroy(s);
+    }
+    int status() {
+        int st = 0;
+        BL_HIP(hipMemcpyAsync(&st, w.misc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
+        BL_HIP(hipStreamSynchronize(s));
+        return (st & ST_DUPKEY) ? BL_E_DUPKEY : BL_OK;
+    }
+};
+
 }  // namespace
 
 int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode, const float *model,
                   hipStream_t s) {
-    (void)model;
-    const int g64 = ((n + 63) / 64 + WG / 64 - 1) / (WG / 64);
     if (mode == FAR_EXACT) {
         // by vertex id, straight from the positions
         hipLaunchKernelGGL(k_far_exact, dim3((n + WG - 1) / WG), dim3(WG), 0, s, d_pos, nullptr, n, d_pos, n,
@@ -218,81 +479,80 @@ int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode
         BL_HIP(hipGetLastError());
         return BL_OK;
     }
-    if (mode != FAR_MONO) return BL_E_ARG;
-    // the tree of d_pos is already in w (bh_build_tree)
-    int rc = ensure_far(w, n);
+    // the tree of d_pos is already in w (bh_build_tree); queries in Morton order
+    int rc = far_cells_of(w, n, s);
     if (rc) return rc;
-    BHNode *nodes = w.node_child.as<BHNode>();
-    FarCell *cells = w.far_cells.as<FarCell>();
-    const int *misc = w.misc.as<int>();
-    if (n >= 2) {
-        BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
-        hipLaunchKernelGGL(k_far_moments, dim3((n + WG - 1) / WG), dim3(WG), 0, s, nodes, w.spos.as<double2>(), misc,
-                           cells, w.far_arrive.as<int>());
-    }
-    hipLaunchKernelGGL(k_far_trav, dim3(g64), dim3(WG), 0, s, w.spos.as<double2>(), w.idx_a.as<int>(), n,
-                       w.spos.as<double2>(), nodes, cells, misc, w.frame.as<double>(), theta, w.rep.as<double2>());
-    BL_HIP(hipGetLastError());
-    return BL_OK;
+    return launch_trav(w, w.spos.as<double2>(), w.idx_a.as<int>(), n, theta, mode, model, w.rep.as<double2>(), s);
 }
 
 // Host entry: far-field forces of nq query points against the n points xy.
-int far_forces_host(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
-                    double *out) {
-    hipStream_t s = nullptr;
-    BL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    struct Guard {
-        hipStream_t s;
-        BHWork w;
-        DevBuf pts, qs, qi, res;
-        ~Guard() {
-            (void)hipStreamSynchronize(s);
-            w.release();
-            pts.release();
-            qs.release();
-            qi.release();
-            res.release();
-            (void)hipStreamDestroy(s);
-        }
-    } G{s, {}, {}, {}, {}, {}};
+int far_forces_host(int32_t n, const double *xy, double theta, int32_t mode, const float *model, int32_t nq,
+                    const double *q, double *out) {
+    HostRun G;
+    BL_HIP(hipStreamCreateWithFlags(&G.s, hipStreamNonBlocking));
     if (G.pts.ensure(16 * (size_t)n) || G.qs.ensure(16 * (size_t)std::max(nq, 1)) ||
-        G.qi.ensure(4 * (size_t)std::max(nq, 1)) || G.res.ensure(16 * (size_t)std::max(nq, 1)))
+        G.qi.ensure(4 * (size_t)std::max(nq, 1)) || G.res.ensure(16 * (size_t)std::max(nq, 1)) ||
+        G.mdl.ensure(4 * (size_t)MLP_NPARAM))
         return BL_E_NOMEM;
-    BL_HIP(hipMemcpyAsync(G.pts.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, s));
     if (nq == 0) return BL_OK;
-    BL_HIP(hipMemcpyAsync(G.qs.p, q, 16 * (size_t)nq, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(G.pts.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, G.s));
+    BL_HIP(hipMemcpyAsync(G.qs.p, q, 16 * (size_t)nq, hipMemcpyHostToDevice, G.s));
     std::vector<int> ids(nq);
     for (int i = 0; i < nq; ++i) ids[i] = i;
-    BL_HIP(hipMemcpyAsync(G.qi.p, ids.data(), 4 * (size_t)nq, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemcpyAsync(G.qi.p, ids.data(), 4 * (size_t)nq, hipMemcpyHostToDevice, G.s));
     if (mode == FAR_EXACT) {
-        hipLaunchKernelGGL(k_far_exact, dim3((nq + WG - 1) / WG), dim3(WG), 0, s, G.qs.as<double2>(), G.qi.as<int>(),
+        hipLaunchKernelGGL(k_far_exact, dim3((nq + WG - 1) / WG), dim3(WG), 0, G.s, G.qs.as<double2>(), G.qi.as<int>(),
                            nq, G.pts.as<double2>(), n, G.res.as<double2>());
         BL_HIP(hipGetLastError());
-    } else if (mode == FAR_MONO) {
-        int rc = bh_build_tree(G.w, G.pts.as<double2>(), n, s);
-        if (rc) return rc;
-        rc = ensure_far(G.w, n);
-        if (rc) return rc;
-        const int *misc = G.w.misc.as<int>();
-        if (n >= 2) {
-            BL_HIP(hipMemsetAsync(G.w.far_arrive.p, 0, 4 * (size_t)n, s));
-            hipLaunchKernelGGL(k_far_moments, dim3((n + WG - 1) / WG), dim3(WG), 0, s, G.w.node_child.as<BHNode>(),
-                               G.w.spos.as<double2>(), misc, G.w.far_cells.as<FarCell>(), G.w.far_arrive.as<int>());
-        }
-        const int g64 = ((nq + 63) / 64 + WG / 64 - 1) / (WG / 64);
-        hipLaunchKernelGGL(k_far_trav, dim3(g64), dim3(WG), 0, s, G.qs.as<double2>(), G.qi.as<int>(), nq,
-                           G.w.spos.as<double2>(), G.w.node_child.as<BHNode>(), G.w.far_cells.as<FarCell>(), misc,
-                           G.w.frame.as<double>(), theta, G.res.as<double2>());
-        BL_HIP(hipGetLastError());
-        int status = 0;
-        BL_HIP(hipMemcpyAsync(&status, misc, sizeof(int), hipMemcpyDeviceToHost, s));
-        BL_HIP(hipStreamSynchronize(s));
-        if (status & ST_DUPKEY) return BL_E_DUPKEY;
     } else {
-        return BL_E_ARG;
+        if (mode == FAR_LEARNED) {
+            if (!model) return BL_E_ARG;
+            BL_HIP(hipMemcpyAsync(G.mdl.p, model, 4 * (size_t)MLP_NPARAM, hipMemcpyHostToDevice, G.s));
+        }
+        int rc = bh_build_tree(G.w, G.pts.as<double2>(), n, G.s);
+        if (rc) return rc;
+        rc = far_cells_of(G.w, n, G.s);
+        if (rc) return rc;
+        rc = launch_trav(G.w, G.qs.as<double2>(), G.qi.as<int>(), nq, theta, mode, G.mdl.as<float>(),
+                         G.res.as<double2>(), G.s);
+        if (rc) return rc;
+        rc = G.status();
+        if (rc) return rc;
     }
-    BL_HIP(hipMemcpyAsync(out, G.res.p, 16 * (size_t)nq, hipMemcpyDeviceToHost, s));
-    BL_HIP(hipStreamSynchronize(s));
+    BL_HIP(hipMemcpyAsync(out, G.res.p, 16 * (size_t)nq, hipMemcpyDeviceToHost, G.s));
+    BL_HIP(hipStreamSynchronize(G.s));
+    return BL_OK;
+}
+
+// Host entry: training samples of nq query points (cap cells each) against the n points xy.
+int far_samples_host(int32_t n, const double *xy, double theta, int32_t nq, const double *q, int32_t cap,
+                     float *feat, float *targ, int32_t *count) {
+    HostRun G;
+    BL_HIP(hipStreamCreateWithFlags(&G.s, hipStreamNonBlocking));
+    const size_t rows = (size_t)nq * (size_t)cap;
+    if (G.pts.ensure(16 * (size_t)n) || G.qs.ensure(16 * (size_t)std::max(nq, 1)) ||
+        G.f.ensure(4 * MLP_IN * std::max<size_t>(rows, 1)) || G.t.ensure(8 * std::max<size_t>(rows, 1)) ||
+        G.c.ensure(4 * (size_t)std::max(nq, 1)))
+        return BL_E_NOMEM;
+    if (nq == 0) return BL_OK;
+    BL_HIP(hipMemcpyAsync(G.pts.p, xy, 16 * (size_t)n, hipMemcpyHostToDevice, G.s));
+    BL_HIP(hipMemcpyAsync(G.qs.p, q, 16 * (size_t)nq, hipMemcpyHostToDevice, G.s));
+    int rc = bh_build_tree(G.w, G.pts.as<double2>(), n, G.s);
+    if (rc) return rc;
+    rc = far_cells_of(G.w, n, G.s);
+    if (rc) return rc;
+    BL_HIP(hipMemsetAsync(G.c.p, 0, 4 * (size_t)nq, G.s));
+    const int g64 = ((nq + 63) / 64 + WG / 64 - 1) / (WG / 64);
+    hipLaunchKernelGGL(k_far_samples, dim3(g64), dim3(WG), 0, G.s, G.qs.as<double2>(), nq, G.w.spos.as<double2>(),
+                       G.w.node_child.as<BHNode>(), G.w.far_cells.as<FarCell>(), G.w.misc.as<int>(),
+                       G.w.frame.as<double>(), theta, cap, G.f.as<float>(), G.t.as<float>(), G.c.as<int>());
+    BL_HIP(hipGetLastError());
+    rc = G.status();
+    if (rc) return rc;
+    BL_HIP(hipMemcpyAsync(feat, G.f.p, 4 * MLP_IN * rows, hipMemcpyDeviceToHost, G.s));
+    BL_HIP(hipMemcpyAsync(targ, G.t.p, 8 * rows, hipMemcpyDeviceToHost, G.s));
+    BL_HIP(hipMemcpyAsync(count, G.c.p, 4 * (size_t)nq, hipMemcpyDeviceToHost, G.s));
+    BL_HIP(hipStreamSynchronize(G.s));
     return BL_OK;
 }
 

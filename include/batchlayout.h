@@ -50,6 +50,9 @@ extern "C" {
 #define BL_ALGO_FAR_MONO 40  /* cells accepted by side^2 < bht^2 |c - v|^2 add their
                                 mass-weighted monopole m (c - v)/|c - v|^2 */
 #define BL_ALGO_FAR_EXACT 41 /* the exact sum over all vertices (O(n^2) per iteration) */
+#define BL_ALGO_FAR_LEARNED 42 /* monopoles corrected by the learned far-field model set with
+                                  bl_graph_set_far_model (BL_E_ARG without one) */
+#define BL_ALGO_FAR_QUAD 43  /* monopoles corrected by the analytic second-order multipole term */
 
 /* ---- precision of the pair arithmetic -------------------------------------------- */
 #define BL_PREC_MIXED 0      /* default: FP64 differences and sums; 1/|d|^2 seeded by the FP32
@@ -150,13 +153,53 @@ int bl_bh_forces(int32_t n, const double *xy, double theta, int32_t nq, const do
                  double *out, int32_t precision);
 
 /* Repulsion of the physically consistent layouts (no reference counterpart) on the device:
- * out[k] = approximation of sum_j (x_j - q_k)/|x_j - q_k|^2 over the n points xy.
- * mode BL_FAR_MONO: mass-weighted monopoles of the cells with side^2 < theta^2 |c - q|^2 on
- * the compressed quadtree of xy (single points exactly); BL_FAR_EXACT: the exact sum. */
+ * out[k] = approximation of sum_j (x_j - q_k)/|x_j - q_k|^2 over the n points xy, on the
+ * compressed quadtree of xy (single points exactly; a cell is accepted when its side^2 <
+ * theta^2 |c - q|^2, c = mean of its m points).  Seen from q, an accepted cell's exact field is
+ * (m/rho)((1 + a) u + b w) with rho = |c - q|, u = (c - q)/rho, w = u turned by +90 degrees:
+ *   BL_FAR_MONO     a = b = 0 (mass-weighted monopole)
+ *   BL_FAR_QUAD     a = S_uu - S_ww, b = -2 S_uw (second-order multipole; S = the cell's central
+ *                   second moments / (m rho^2) in the (u, w) frame)
+ *   BL_FAR_LEARNED  (a, b) = (side/rho)^2 * model(features), model = BL_FAR_MODEL_PARAMS floats
+ *   BL_FAR_EXACT    the exact sum (theta ignored) */
 #define BL_FAR_MONO 1
 #define BL_FAR_EXACT 2
-int bl_far_forces(int32_t n, const double *xy, double theta, int32_t mode, int32_t nq, const double *q,
-                  double *out);
+#define BL_FAR_LEARNED 3
+#define BL_FAR_QUAD 4
+int bl_far_forces(int32_t n, const double *xy, double theta, int32_t mode, const float *model, int32_t nq,
+                  const double *q, double *out);
+
+/* ---- learned far field (no reference counterpart) ----------------------------------- */
+/* The model: 8 features -> 64 -> 64 -> 64 -> 2, GELU (tanh form), evaluated with BF16 weights
+ * and activations on the matrix cores, FP32 accumulation.  Parameters (FP32, this order):
+ * W1[64][8] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2].
+ * Features of a cell seen from q: [side/rho, S_uu, S_uw, S_ww, S_uu - S_ww, log2(m)/16, 1/m,
+ * (side/rho)^2]; the model's outputs are (a, b) / (side/rho)^2. */
+#define BL_FAR_MODEL_PARAMS 9026
+#define BL_FAR_FEATURES 8
+
+/* Training data: for each of the nq query points, the cells accepted by the traversal at
+ * theta (at most cap, in traversal order): feat[(k*cap + j)*8 ..] and the exact (a, b) in
+ * targ[(k*cap + j)*2 ..] (the exact field summed over the cell's points in FP64); count[k] =
+ * number of rows of query k. */
+int bl_far_samples(int32_t n, const double *xy, double theta, int32_t nq, const double *q, int32_t cap,
+                   float *feat, float *targ, int32_t *count);
+
+/* Glorot-uniform weights drawn from `seed`, zero biases, output layer scaled by 0.1. */
+int bl_far_model_init(float *params, uint64_t seed);
+/* out[2k..2k+2) = model(feat[8k..8k+8)) on the device. */
+int bl_far_model_eval(const float *params, int32_t n, const float *feat, float *out);
+/* Loss L = (1/n) sum_k |model(feat_k) - targ_k|^2 and its gradient (BL_FAR_MODEL_PARAMS floats),
+ * by the device's forward and backward kernels. */
+int bl_far_model_grad(const float *params, int32_t n, const float *feat, const float *targ, float *loss,
+                      float *grad);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the device for `epochs` passes over a shuffled
+ * order of the n samples (shuffle drawn from seed), `batch` samples per step; params updated in
+ * place; losses[e] (nullable) = mean loss of epoch e.  Deterministic for a given seed. */
+int bl_far_model_train(float *params, int32_t n, const float *feat, const float *targ, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float *losses);
+/* Attaches a copy of the model to the graph for BL_ALGO_FAR_LEARNED (NULL detaches). */
+int bl_graph_set_far_model(bl_graph *g, const float *params);
 
 /* ---- batched small graphs -------------------------------------------------------- */
 /* Lays out `count` independent graphs in one persistent launch (no counterpart in the
