@@ -33,6 +33,7 @@
 // reference's depth-first quadrant order and uses its FP64 operations, so the force is
 // the reference's bit for bit.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -821,6 +822,242 @@ __global__ __launch_bounds__(WALK_T) void k_walk(const int *__restrict__ rowptr,
 }
 
 // ---------------------------------------------------------------------------------------
+// 8b. the batch walk over GW_G workgroups, for batches with many CSR entries.  One
+// workgroup keeps only a few hundred misses in flight, so a batch of tens of thousands of
+// entries with random neighbour positions is bound by that; spreading the entries over
+// GW_G workgroups multiplies it, at the price of one grid-wide barrier per batch:
+//   A. every workgroup takes a contiguous share of the batch's entries (each thread a
+//      contiguous run of them) and sums per row: a row inside one thread -> fbuf; rows
+//      spread over threads are combined in the workgroup by a segmented scan over the
+//      threads' first-row partials (a fixed tree order); a row spread over workgroups
+//      leaves the workgroup's partial of its first or last row (whead / wtail)
+//   barrier
+//   B. every workgroup combines, in workgroup order, the partials of all rows of the batch,
+//      subtracts the repulsion and applies the moves into its own LDS copy (identical in
+//      all workgroups); workgroup 0 also writes them to memory.  The next batch reads the
+//      positions of this batch from that LDS copy, and older positions from memory, which
+//      were written before a barrier that every reader has passed since.
+// The results differ from the one-workgroup walk only in the order of the partial sums.
+constexpr int GW_T = 1024;
+constexpr int GW_G = 32;
+constexpr int GW_MIN_ENTRIES = 4096;  // mean CSR entries per batch from which it is used
+
+struct GridWalk {
+    const int *rowptr, *colids;
+    const float *w_edge;
+    double W;
+    double2 *pos;
+    const double2 *rep;
+    double2 *fbuf;
+    int n, B;
+    double step;
+    double *eterm;
+    const int *status;
+    double2 *whead, *wtail;  // [2][GW_G] by batch parity
+    int *wfirst, *wlast;     // [2][GW_G]
+    unsigned *bar;
+    int *err;
+    long long ticks;
+};
+
+__device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+__global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
+    constexpr int T = GW_T;
+    __shared__ double2 head[T], tail[T], S[T];
+    __shared__ int hrow[T], trow[T];
+    __shared__ int rps[WALK_MAXB + 1];
+    __shared__ double2 newpos[WALK_MAXB];
+    __shared__ double2 bpos[WALK_MAXB];  // the batch's own positions, read before anyone moves them
+    __shared__ double2 wsum[T / 64];
+    __shared__ int wkfirst[T / 64], wklast[T / 64];
+    __shared__ int bad;
+    if (__builtin_amdgcn_readfirstlane(*a.status)) return;
+    const int t = threadIdx.x, w = blockIdx.x, G = gridDim.x;
+    const int lane = t & 63, wv = t >> 6;
+    const int n = a.n, B = a.B;
+    const int nb = (n + B - 1) / B;
+    if (t == 0) bad = 0;
+    int pb0 = 0, pb1 = 0;  // the previous batch, whose moved positions are in newpos
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = b * B, b1 = min(n, b0 + B), bc = b1 - b0, par = b & 1;
+        for (int k = t; k <= bc; k += T) rps[k] = a.rowptr[b0 + k];
+        for (int k = t; k < bc; k += T) bpos[k] = a.pos[b0 + k];
+        __syncthreads();
+        auto row_of = [&](int entry) {  // last r in [b0, b1) with rowptr[r] <= entry
+            int lo = b0, hi = b1;
+            while (hi - lo > 1) {
+                const int m = (lo + hi) >> 1;
+                if (rps[m - b0] <= entry) lo = m; else hi = m;
+            }
+            return lo;
+        };
+        const int E0 = rps[0], E1 = rps[bc], total = E1 - E0;
+        const int Cw = (total + G - 1) / G;
+        const int Sw = min(E1, E0 + w * Cw), Ew = min(E1, Sw + Cw);
+        const int c = max(1, (Cw + T - 1) / T);
+        const int s = min(Ew, Sw + t * c), e = min(Ew, s + c);
+        // A1. thread partials
+        int hr = -1, tr = -1;
+        double2 hv = make_double2(0.0, 0.0), tv = make_double2(0.0, 0.0);
+        if (s < e) {
+            int r = row_of(s);
+            const int first = r;
+            int rend = rps[r + 1 - b0];
+            double2 pi = bpos[r - b0];
+            double ax = 0.0, ay = 0.0;
+            for (int k0 = s; k0 < e; k0 += WALK_G) {
+                int cid[WALK_G];
+                double2 pj[WALK_G];
+                double wk[WALK_G];
+#pragma unroll
+                for (int g = 0; g < WALK_G; ++g) cid[g] = k0 + g < e ? a.colids[k0 + g] : -1;
+#pragma unroll
+                for (int g = 0; g < WALK_G; ++g) {
+                    const int j = cid[g];
+                    pj[g] = j < 0 ? make_double2(0.0, 0.0) : (j >= pb0 && j < pb1) ? newpos[j - pb0] : a.pos[j];
+                    wk[g] = (a.w_edge && k0 + g < e) ? a.W * (double)a.w_edge[k0 + g] : a.W;
+                }
+#pragma unroll
+                for (int g = 0; g < WALK_G; ++g) {
+                    const int k = k0 + g;
+                    if (k >= e) break;
+                    while (k >= rend) {  // row r complete inside this thread's run
+                        if (r == first) {
+                            hr = r;
+                            hv = make_double2(ax, ay);
+                        } else {
+                            a.fbuf[r] = make_double2(ax, ay);
+                        }
+                        ++r;
+                        rend = rps[r + 1 - b0];
+                        pi = bpos[r - b0];
+                        ax = ay = 0.0;
+                    }
+                    const double dx = pj[g].x - pi.x, dy = pj[g].y - pi.y;
+                    const double sc = wk[g] * sqrt(dx * dx + dy * dy);
+                    ax = fma(dx, sc, ax);
+                    ay = fma(dy, sc, ay);
+                }
+            }
+            if (r == first) {
+                hr = r;
+                hv = make_double2(ax, ay);
+            } else {
+                tr = r;
+                tv = make_double2(ax, ay);
+            }
+        }
+        hrow[t] = hr;
+        trow[t] = tr;
+        head[t] = hv;
+        tail[t] = tv;
+        // A2. segmented inclusive scan of the first-row partials over the threads (runs of
+        // equal hrow are contiguous): in the wave by shuffles, then across waves
+        double2 v = hv;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double px = __shfl_up(v.x, off, 64), py = __shfl_up(v.y, off, 64);
+            const int kp = __shfl_up(hr, off, 64);
+            if (lane >= off && kp == hr) {
+                v.x += px;
+                v.y += py;
+            }
+        }
+        if (lane == 63) {
+            wsum[wv] = v;
+            wklast[wv] = hr;
+        }
+        if (lane == 0) wkfirst[wv] = hr;
+        __syncthreads();
+        double2 carry = make_double2(0.0, 0.0);
+        for (int q = wv - 1; q >= 0; --q) {  // earlier waves of the same run, nearest first
+            if (wklast[q] != hr) break;
+            carry = add2(carry, wsum[q]);
+            if (wkfirst[q] != hr) break;
+        }
+        S[t] = add2(carry, v);
+        __syncthreads();
+        // A3. rows of this workgroup's share: complete ones -> fbuf, the first and last
+        // (when they continue in other workgroups) -> whead / wtail
+        if (Sw < Ew) {
+            const int rf = row_of(Sw), rl = row_of(Ew - 1);
+            if (t == 0) {
+                a.wfirst[par * GW_G + w] = rf;
+                a.wlast[par * GW_G + w] = rl != rf ? rl : -1;
+            }
+            for (int i = rf + t; i <= rl; i += T) {
+                const int r0 = rps[i - b0], r1 = rps[i + 1 - b0];
+                const int rs = max(r0, Sw), re = min(r1, Ew);
+                if (re <= rs) continue;
+                const int ta = (rs - Sw) / c, tb = (re - 1 - Sw) / c;
+                if (ta == tb && hrow[ta] != i && trow[ta] != i) continue;  // already in fbuf
+                const double2 val = hrow[ta] == i ? S[tb] : (tb > ta ? add2(tail[ta], S[tb]) : tail[ta]);
+                if (r0 >= Sw && r1 <= Ew) {
+                    a.fbuf[i] = val;
+                } else if (i == rf) {
+                    a.whead[par * GW_G + w] = val;
+                } else {
+                    a.wtail[par * GW_G + w] = val;
+                }
+            }
+        }
+        // grid barrier: every workgroup's share of the batch is summed
+        __threadfence();
+        __syncthreads();
+        if (t == 0) {
+            __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned target = (unsigned)G * (unsigned)(b + 1);
+            const long long t0 = wall_clock64();
+            while (__hip_atomic_load(a.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    bad = 1;
+                    break;
+                }
+                if (wall_clock64() - t0 > a.ticks) {
+                    atomicOr(a.err, 1);
+                    bad = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+        // the same value in every thread: a uniform exit, not a per-lane branch out of the loop
+        if (__builtin_amdgcn_readfirstlane(bad)) return;
+        // B. combine in workgroup order, subtract the repulsion, move (every workgroup)
+        for (int i = b0 + t; i < b1; i += T) {
+            const int r0 = rps[i - b0], r1 = rps[i + 1 - b0];
+            double2 f = make_double2(0.0, 0.0);
+            if (r1 > r0) {
+                const int wa = (r0 - E0) / Cw, wb = (r1 - 1 - E0) / Cw;
+                if (wa == wb) {
+                    f = a.fbuf[i];
+                } else {
+                    f = a.wfirst[par * GW_G + wa] == i ? a.whead[par * GW_G + wa] : a.wtail[par * GW_G + wa];
+                    for (int q = wa + 1; q <= wb; ++q) f = add2(f, a.whead[par * GW_G + q]);
+                }
+            }
+            const double2 rr = a.rep[i];
+            f.x -= rr.x;  // f - calcRepForce(x_i), :1059
+            f.y -= rr.y;
+            const double mag = sqrt(f.x * f.x + f.y * f.y);
+            double2 q = bpos[i - b0];
+            q.x = q.x + (f.x / mag) * a.step;
+            q.y = q.y + (f.y / mag) * a.step;
+            newpos[i - b0] = q;
+            if (w == 0) {
+                a.pos[i] = q;
+                a.eterm[i] = f.x * f.x + f.y * f.y;
+            }
+        }
+        __syncthreads();
+        pb0 = b0;
+        pb1 = b1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // host orchestration
 struct TreeBufs {
     uint64_t *keys;  // sorted keys
@@ -924,6 +1161,57 @@ int launch_rep(BHWork &w, int n, double theta, int precision, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+// layout of BHWork::gwalk: whead[2G] wtail[2G] (double2), wfirst[2G] wlast[2G] (int), bar, err
+constexpr size_t GW_BYTES = 2 * (2 * GW_G * sizeof(double2)) + 2 * (2 * GW_G * sizeof(int)) + 2 * sizeof(int);
+int *gwalk_err(BHWork &w) {
+    return reinterpret_cast<int *>(w.gwalk.as<char>() + GW_BYTES - sizeof(int));
+}
+
+// Launches one iteration's walk on GW_G co-resident workgroups; anything but BL_OK means
+// nothing was launched (the caller uses the one-workgroup walk).
+int launch_walk_grid(bl_graph *g, double2 *pos, const bl_params *p, double step, hipStream_t s) {
+    BHWork &w = g->bh;
+    if (w.gwalk.ensure(GW_BYTES)) return BL_E_NOMEM;
+    char *base = w.gwalk.as<char>();
+    GridWalk a;
+    a.rowptr = g->rowptr.as<int>();
+    a.colids = g->colids.as<int>();
+    a.w_edge = g->has_w ? g->w_edge.as<float>() : nullptr;
+    a.W = p->weight;
+    a.pos = pos;
+    a.rep = w.rep.as<double2>();
+    a.fbuf = g->scratch.as<double2>();
+    a.n = g->n;
+    a.B = p->batch;
+    a.step = step;
+    a.eterm = g->eterm.as<double>();
+    a.status = w.misc.as<int>();
+    a.whead = reinterpret_cast<double2 *>(base);
+    a.wtail = a.whead + 2 * GW_G;
+    a.wfirst = reinterpret_cast<int *>(a.wtail + 2 * GW_G);
+    a.wlast = a.wfirst + 2 * GW_G;
+    a.bar = reinterpret_cast<unsigned *>(a.wlast + 2 * GW_G);
+    a.err = reinterpret_cast<int *>(a.bar + 1);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g->device) != hipSuccess || khz <= 0)
+        khz = 100000;
+    a.ticks = (long long)khz * 2000;  // 2 s per barrier wait
+    // clears the barrier counter, and the error flag unless an earlier iteration of this run set it
+    if (hipMemsetAsync(a.bar, 0, sizeof(unsigned), s) != hipSuccess) return BL_E_DEVICE;
+    if (!w.gwalk_used && hipMemsetAsync(a.err, 0, sizeof(int), s) != hipSuccess) return BL_E_DEVICE;
+    void *args[] = {&a};
+    const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_walk_grid), dim3(GW_G),
+                                                    dim3(GW_T), args, 0, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return BL_E_DEVICE;
+    }
+    w.gwalk_used = 1;
+    return BL_OK;
+}
+}  // namespace
+
 int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s) {
     return build_tree(w, d_pos, n, s, nullptr, nullptr);
 }
@@ -942,6 +1230,18 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     prof.on = p->profile != 0;
     prof.s = s;
     double STEP = step0;
+    // the batch walk over several workgroups pays off for batches with many entries;
+    // BL_WALK=grid|single overrides the choice
+    const int nbt = (n + p->batch - 1) / p->batch;
+    bool use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB &&
+                    (double)g->nnz / (double)nbt >= (double)GW_MIN_ENTRIES;
+    if (const char *wenv = getenv("BL_WALK")) {
+        if (!strcmp(wenv, "grid")) use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB;
+        else if (!strcmp(wenv, "single")) use_grid = false;
+    }
+    g->bh.gwalk_used = 0;
+    // the exact mode builds no tree, so nothing else clears the status word the walk reads
+    if (far_mode == FAR_EXACT) BL_HIP(hipMemsetAsync(g->bh.misc.p, 0, 16, s));
     for (int it = 0; it < iters; ++it) {
         if (far_mode != FAR_EXACT) {
             prof.begin(3);
@@ -960,7 +1260,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
                                g->bh.misc.as<int>());
-        else
+        else if (!(use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK))
             hipLaunchKernelGGL(p->batch <= WALK_MAXB ? k_walk<true> : k_walk<false>, dim3(1), dim3(WALK_T), 0, s,
                                g->rowptr.as<int>(), g->colids.as<int>(), g->has_w ? g->w_edge.as<float>() : nullptr,
                                p->weight, pos, g->bh.rep.as<double2>(), g->scratch.as<double2>(), n, p->batch,
@@ -980,8 +1280,11 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         BL_HIP(hipMemcpyAsync(&status, g->bh.misc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
         BL_HIP(hipMemcpyAsync(&E, g->energy.as<double>(), sizeof(double), hipMemcpyDeviceToHost, s));
     }
+    int gerr = 0;
+    if (g->bh.gwalk_used) BL_HIP(hipMemcpyAsync(&gerr, gwalk_err(g->bh), sizeof(int), hipMemcpyDeviceToHost, s));
     BL_HIP(hipStreamSynchronize(s));
     if (status & ST_DUPKEY) return BL_E_DUPKEY;
+    if (gerr) return BL_E_DEVICE;  // a grid barrier timed out (the walk stopped early)
     if (st) {
         st->energy = E;
         st->iters_done = iters;

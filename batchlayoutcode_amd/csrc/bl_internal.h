@@ -43,6 +43,8 @@ struct BHWork {
     DevBuf rep;                              // repulsion result per vertex (double2)
     DevBuf misc;                             // counters / status words
     DevBuf far_cells, far_arrive;            // physically consistent far field: cell summaries
+    DevBuf gwalk;                            // multi-workgroup walk: partials, barrier, error flag
+    int gwalk_used = 0;                      // the last run launched it
     int32_t n_alloc = 0;
     void release();
 };
@@ -95,6 +97,24 @@ struct Prof {
     void end() {
         if (!on) return;
         (void)hipEventRecord(ev[2 * used + 1], s);
+        ++used;
+    }
+    // Events for one launch made with hipExtLaunchKernelGGL, which records them in the
+    // dispatch itself (start and end of the kernel, without the gaps of separate event
+    // records).  Sets *a, *b to null when profiling is off.
+    void ext(int c, hipEvent_t *a, hipEvent_t *b) {
+        *a = *b = nullptr;
+        if (!on) return;
+        if (2 * (used + 1) > ev.size()) {
+            hipEvent_t x, y;
+            if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) { on = false; return; }
+            ev.push_back(x);
+            ev.push_back(y);
+            cls.push_back(c);
+        }
+        cls[used] = c;
+        *a = ev[2 * used];
+        *b = ev[2 * used + 1];
         ++used;
     }
     // waits for the recorded launches and folds their durations into ms/cnt

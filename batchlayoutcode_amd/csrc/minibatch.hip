@@ -37,6 +37,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "bl_internal.h"
 #include "pair_math.h"
 
@@ -572,17 +574,17 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
                 } else {
                     const int ipw = WG / JS;
                     const dim3 grid(G + (bc + 3) / 4, (bc + ipw - 1) / ipw);
-                    prof.begin(0);
+                    hipEvent_t e0, e1;
+                    prof.ext(0, &e0, &e1);
                     if (p->precision == BL_PREC_FP32)
-                        hipLaunchKernelGGL((k_pairs32<4, false>), grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G,
-                                           part, P, erp, eci, ew, p->weight, law);
+                        hipExtLaunchKernelGGL((k_pairs32<4, false>), grid, dim3(WG), 0, s, e0, e1, 0, pos, n, b0, bc,
+                                              chunk, G, part, P, erp, eci, ew, p->weight, law);
                     else
-                        hipLaunchKernelGGL((k_pairs32<4, true>), grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G,
-                                           part, P, erp, eci, ew, p->weight, law);
-                    prof.end();
-                    prof.begin(1);
-                    hipLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, part, P, pos, b0, bc, STEP, eterm);
-                    prof.end();
+                        hipExtLaunchKernelGGL((k_pairs32<4, true>), grid, dim3(WG), 0, s, e0, e1, 0, pos, n, b0, bc,
+                                              chunk, G, part, P, erp, eci, ew, p->weight, law);
+                    prof.ext(1, &e0, &e1);
+                    hipExtLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, e0, e1, 0, part, P, pos, b0, bc, STEP,
+                                          eterm);
                 }
             }
         }

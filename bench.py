@@ -17,6 +17,11 @@ Also reported:
                 against the MI355X FP64 vector peak (78.6 TFLOP/s; the default pair arithmetic
                 is FP64 apart from the FP32 reciprocal seed)
   fp32_mode     the same workload with FP32 |d|^2 and reciprocal (against 157.3 TFLOP/s)
+  far_field     the physically consistent Barnes-Hut layouts (algo 2's d/|d|^2 repulsion for
+                every pair) on the same graph: exact sum, mass-weighted monopole, analytic
+                quadrupole and the learned correction (model trained first on cells from
+                layouts of small synthetic graphs), with the reference's unit-vector BH
+                (algo 4) beside them; vertex-updates/s and energy after 10 iterations
   cpu_baseline  the reference built from its own sources (oracle/_ref/ref_fast) or, when
                 that binary is absent, the C oracle, timed on this host's cores for one
                 iteration of the same workload
@@ -81,6 +86,36 @@ def cpu_baseline(n, rp, ci, init, batch, threads):
                        f"{threads} threads; {secs:.2f} s")
 
 
+def far_leg(api, graphs, rp, ci, init, iters, theta=1.0):
+    """Physically consistent Barnes-Hut on the same graph: exact repulsion, monopole,
+    analytic quadrupole and learned far field; vertex-updates/s and final energy vs exact."""
+    from batchlayoutcode_amd import farmodel
+
+    n = len(rp) - 1
+    model, info = farmodel.train(theta=theta)
+    log(f"far field: {info['training_cells']} training cells, loss {info['loss_first']:.3e} -> "
+        f"{info['loss_last']:.3e} ({info['data_seconds'] + info['train_seconds']:.1f} s)")
+    res = {}
+    with api.Graph(rp, ci) as g:
+        g.set_far_model(model)
+        for name, algo in (("exact", api.ALGO_FAR_EXACT), ("monopole", api.ALGO_FAR_MONO),
+                           ("quadrupole", api.ALGO_FAR_QUAD), ("learned", api.ALGO_FAR_LEARNED),
+                           ("reference_bh_unit_vector", api.ALGO_BH)):
+            g.run(init, algo=algo, iters=1, batch=256, bht=theta)  # warm-up
+            _, st = g.run(init, algo=algo, iters=iters, batch=256, bht=theta)
+            _, sp = g.run(init, algo=algo, iters=iters, batch=256, bht=theta, profile=1)
+            res[name] = dict(vertex_updates_per_s=n * iters / st.seconds, energy=st.energy,
+                             repulsion_ms_per_iter=sp.kernel_ms[0] / iters,
+                             tree_ms_per_iter=sp.kernel_ms[3] / iters, walk_ms_per_iter=sp.kernel_ms[1] / iters)
+            log(f"far field {name}: {res[name]['vertex_updates_per_s']:.3e} v-u/s, E={st.energy:.6e}, "
+                f"repulsion {res[name]['repulsion_ms_per_iter']:.3f} ms/it")
+    e_ex = res["exact"]["energy"]
+    for name in ("monopole", "quadrupole", "learned"):
+        res[name]["energy_rel_diff_vs_exact"] = abs(res[name]["energy"] - e_ex) / e_ex
+    return dict(theta=theta, iterations=iters, batch=256, training=info,
+                model="8-64-64-64-2 GELU, BF16 MFMA, FP32 accumulation", modes=res)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +124,12 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--no-far", action="store_true", help="skip the Barnes-Hut / far-field comparison")
+    ap.add_argument("--minimal", action="store_true",
+                    help="only the timed region (for counter collection under a profiler); no JSON line")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a plain device copy of 256 MiB (5 times): the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -105,6 +146,14 @@ def main():
     log(f"graph: n={n} nnz={len(ci)} ({time.time() - t0:.1f} s); device: {api.device_info()}")
 
     dev = torch.device("cuda:0")
+    if args.calib_copy:
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("calibration copy: 5 x 256 MiB device-to-device")
     d_init = torch.from_numpy(init).to(dev)
     d_xy = torch.empty_like(d_init)
     stream = torch.cuda.current_stream(dev)
@@ -130,6 +179,8 @@ def main():
         # timed region (default precision)
         secs, st = timed(g, api.PREC_MIXED)
         log(f"timed: {K} iterations in {secs:.3f} s, E={st.energy:.6e}")
+        if args.minimal:
+            return
         # profiled pass: per-launch HIP events on the same stream
         _, sp_prof = timed(g, api.PREC_MIXED, profile=1)
         pair_ms = sp_prof.kernel_ms[0] / max(1, sp_prof.kernel_launches[0])
@@ -172,6 +223,8 @@ def main():
                roofline=roofline, energy=energy,
                fp32_mode=dict(value=n * K / secs32, ms_per_step=secs32 * 1e3 / K, pair_kernel_us=pair32_ms * 1e3,
                               achieved_tflops=achieved32, frac_of_fp32_peak=achieved32 / PEAK_FP32_TFLOPS))
+    if not args.no_far:
+        out["far_field"] = far_leg(api, graphs, rp, ci, init, iters=10)
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         log(f"cpu baseline on {threads} threads ...")
