@@ -821,6 +821,194 @@ __global__ __launch_bounds__(WALK_T) void k_walk(const int *__restrict__ rowptr,
     }
 }
 
+// The one-workgroup walk with the batch's first entries prefetched (batches of at most
+// WALK_MAXB vertices).  Besides the staged rowptr slice, own positions and repulsion, each
+// thread loads during batch b the column ids, weights and neighbour positions of its first
+// WALK_PF entries of batch b+1.  A neighbour in batch b is moved after that load: at the
+// start of batch b+1 such entries take the moved position from the LDS copy the move phase
+// keeps (newpos), every other neighbour's position is final for the iteration.  A batch
+// whose entries fit in WALK_PF per thread thus reads no memory on its critical path.
+// Same sums, in the same order, as k_walk.
+constexpr int WALK_PF = 4;
+
+__global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowptr, const int *__restrict__ colids,
+                                                     const float *__restrict__ w_edge, double W,
+                                                     double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                     int n, int B, double step, double *__restrict__ eterm,
+                                                     const int *__restrict__ status) {
+    constexpr int T = WALK_T;
+    constexpr int PER = (WALK_MAXB + T - 1) / T;  // staged vertices per thread
+    __shared__ double2 head[T], tail[T];
+    __shared__ int hrow[T], trow[T];
+    __shared__ int rps[WALK_MAXB + 1];
+    __shared__ double2 bpos[WALK_MAXB], rsum[WALK_MAXB], newpos[WALK_MAXB];
+    if (__builtin_amdgcn_readfirstlane(*status)) return;
+    const int t = threadIdx.x;
+    const int nb = (n + B - 1) / B;
+    int nrp[PER + 1];
+    double2 npos[PER], nrep[PER];
+    int pcid[WALK_PF];
+    double2 ppos[WALK_PF];
+    double pw[WALK_PF];
+    auto fetch = [&](int b) {  // staged data of batch b
+        const int b0 = b * B, bc = min(n, b0 + B) - b0;
+#pragma unroll
+        for (int j = 0; j <= PER; ++j) {
+            const int k = t + j * T;
+            nrp[j] = k <= bc ? rowptr[b0 + k] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int k = t + j * T;
+            npos[j] = k < bc ? pos[b0 + k] : make_double2(0.0, 0.0);
+            nrep[j] = k < bc ? rep[b0 + k] : make_double2(0.0, 0.0);
+        }
+    };
+    auto prefetch = [&](int b) {  // first entries of this thread's share of batch b
+        const int b0 = b * B, b1 = min(n, b0 + B);
+        const int E0 = __builtin_amdgcn_readfirstlane(rowptr[b0]);
+        const int E1 = __builtin_amdgcn_readfirstlane(rowptr[b1]);
+        const int c = max(1, (E1 - E0 + T - 1) / T);
+        const int s = E0 + t * c, e = min(E1, s + c);
+#pragma unroll
+        for (int g = 0; g < WALK_PF; ++g) pcid[g] = s + g < e ? colids[s + g] : -1;
+#pragma unroll
+        for (int g = 0; g < WALK_PF; ++g) {
+            ppos[g] = pcid[g] >= 0 ? pos[pcid[g]] : make_double2(0.0, 0.0);
+            pw[g] = (w_edge && s + g < e) ? W * (double)w_edge[s + g] : W;
+        }
+    };
+    fetch(0);
+    prefetch(0);
+    int pb0 = 0, pb1 = 0;  // the previous batch, whose moved positions are in newpos
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = b * B, b1 = min(n, b0 + B), bc = b1 - b0;
+#pragma unroll
+        for (int j = 0; j <= PER; ++j) {
+            const int k = t + j * T;
+            if (k <= bc) rps[k] = nrp[j];
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int k = t + j * T;
+            if (k < bc) bpos[k] = npos[j];
+        }
+        // prefetched neighbours that the previous batch moved
+#pragma unroll
+        for (int g = 0; g < WALK_PF; ++g)
+            if (pcid[g] >= pb0 && pcid[g] < pb1) ppos[g] = newpos[pcid[g] - pb0];
+        __syncthreads();
+        const int E0 = rps[0], E1 = rps[bc];
+        const int total = E1 - E0;
+        const int c = max(1, (total + T - 1) / T);
+        const int s = E0 + t * c, e = min(E1, s + c);
+        hrow[t] = -1;
+        trow[t] = -1;
+        if (s < e) {
+            int lo = b0, hi = b1;  // row containing entry s: last r with rowptr[r] <= s
+            while (hi - lo > 1) {
+                const int m = (lo + hi) >> 1;
+                if (rps[m - b0] <= s) lo = m; else hi = m;
+            }
+            int r = lo;
+            const int first = r;
+            int rend = rps[r + 1 - b0];
+            double2 pi = bpos[r - b0];
+            double ax = 0.0, ay = 0.0;
+            for (int k0 = s; k0 < e; k0 += WALK_PF) {
+                int cid[WALK_PF];
+                double2 pj[WALK_PF];
+                double wk[WALK_PF];
+                if (k0 == s) {
+#pragma unroll
+                    for (int g = 0; g < WALK_PF; ++g) {
+                        cid[g] = pcid[g];
+                        pj[g] = ppos[g];
+                        wk[g] = pw[g];
+                    }
+                } else {
+#pragma unroll
+                    for (int g = 0; g < WALK_PF; ++g) cid[g] = k0 + g < e ? colids[k0 + g] : -1;
+#pragma unroll
+                    for (int g = 0; g < WALK_PF; ++g) {
+                        pj[g] = cid[g] >= 0 ? pos[cid[g]] : make_double2(0.0, 0.0);
+                        wk[g] = (w_edge && k0 + g < e) ? W * (double)w_edge[k0 + g] : W;
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < WALK_PF; ++g) {
+                    const int k = k0 + g;
+                    if (k >= e) break;
+                    while (k >= rend) {  // row r complete inside this range
+                        if (r == first) {
+                            hrow[t] = r;
+                            head[t] = make_double2(ax, ay);
+                        } else {
+                            rsum[r - b0] = make_double2(ax, ay);
+                        }
+                        ++r;
+                        rend = rps[r + 1 - b0];
+                        pi = bpos[r - b0];
+                        ax = ay = 0.0;
+                    }
+                    const double dx = pj[g].x - pi.x, dy = pj[g].y - pi.y;
+                    const double sc = wk[g] * sqrt(dx * dx + dy * dy);
+                    ax = fma(dx, sc, ax);
+                    ay = fma(dy, sc, ay);
+                }
+            }
+            if (r == first) {
+                hrow[t] = r;
+                head[t] = make_double2(ax, ay);
+            } else {
+                trow[t] = r;
+                tail[t] = make_double2(ax, ay);
+            }
+        }
+        __syncthreads();
+        // the next batch's staged data and first entries, issued before the moves
+        double2 cur_rep[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) cur_rep[j] = nrep[j];
+        if (b + 1 < nb) {
+            fetch(b + 1);
+            prefetch(b + 1);
+        }
+        for (int i = b0 + t, j = 0; i < b1; i += T, ++j) {
+            const int rs = rps[i - b0], re = rps[i + 1 - b0];
+            double2 f = make_double2(0.0, 0.0);
+            if (re > rs) {
+                const int ta = (rs - E0) / c, tb = (re - 1 - E0) / c;
+                if (ta == tb) {
+                    f = hrow[ta] == i ? head[ta] : (trow[ta] == i ? tail[ta] : rsum[i - b0]);
+                } else {
+                    f = trow[ta] == i ? tail[ta] : head[ta];
+                    for (int q = ta + 1; q <= tb; ++q) {
+                        f.x += head[q].x;
+                        f.y += head[q].y;
+                    }
+                }
+            }
+            double2 rr = cur_rep[0];
+#pragma unroll
+            for (int u = 1; u < PER; ++u)
+                if (j == u) rr = cur_rep[u];
+            f.x -= rr.x;  // f - calcRepForce(x_i), :1059
+            f.y -= rr.y;
+            const double mag = sqrt(f.x * f.x + f.y * f.y);
+            double2 q = bpos[i - b0];
+            q.x = q.x + (f.x / mag) * step;
+            q.y = q.y + (f.y / mag) * step;
+            pos[i] = q;
+            newpos[i - b0] = q;
+            eterm[i] = f.x * f.x + f.y * f.y;
+        }
+        __syncthreads();
+        pb0 = b0;
+        pb1 = b1;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // 8b. the batch walk over GW_G workgroups, for batches with many CSR entries.  One
 // workgroup keeps only a few hundred misses in flight, so a batch of tens of thousands of
@@ -1260,11 +1448,17 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
                                g->bh.misc.as<int>());
-        else if (!(use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK))
-            hipLaunchKernelGGL(p->batch <= WALK_MAXB ? k_walk<true> : k_walk<false>, dim3(1), dim3(WALK_T), 0, s,
-                               g->rowptr.as<int>(), g->colids.as<int>(), g->has_w ? g->w_edge.as<float>() : nullptr,
-                               p->weight, pos, g->bh.rep.as<double2>(), g->scratch.as<double2>(), n, p->batch,
-                               STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
+        else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK)
+            ;
+        else if (p->batch <= WALK_MAXB)
+            hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
+                               g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(), n,
+                               p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
+        else
+            hipLaunchKernelGGL(k_walk<false>, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
+                               g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
+                               g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
+                               g->bh.misc.as<int>());
         prof.end();
         prof.begin(2);
         rc = launch_energy(g->eterm.as<double>(), n, g->energy.as<double>(), p->precision, s);

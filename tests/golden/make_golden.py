@@ -14,6 +14,10 @@ Outputs (all small, committed):
                         fast-math build ("xy") and strict-IEEE build ("xy_strict")
   band_<case>.npz       final energies of runs from 1-ulp-perturbed inits (FP64 noise band)
   bh_<case>.npz         Morton scales, sorted permutation, quadtree (pre-order), BH forces
+  out_<case>.txt        the reference CLI's output file (6 significant digits) after k iterations
+  cli_stdout_*.txt      the reference CLI's stdout (line layout read by its Python wrapper)
+  cli_results_*.txt     the row the reference CLI appends to Results.txt
+  cli_init{0,1}_*.txt   output files of -init 0 (DFS placement) and -init 1 (random placement)
 """
 from __future__ import annotations
 
