@@ -16,7 +16,7 @@
 //                  chains collapsed into their lowest node (a chain shares one centroid,
 //                  and the smallest diameter in the chain decides acceptance, so the
 //                  traversal result is the same)
-//   6. k_centroid  unweighted mean of the non-empty quadrants' centroids, bottom-up,
+//   6. k_centroid_* unweighted mean of the non-empty quadrants' centroids, bottom-up,
 //                  summed in quadrant order as :197-217
 //   7. k_rep*      repulsion of every vertex, one wave per 64 Morton-consecutive vertices
 //                  with a shared stack of (node, lane mask): the vertex's own position is
@@ -43,6 +43,7 @@
 
 namespace bl {
 
+// out: 4 + 256 doubles, the sum in out[0] (minibatch.hip)
 int launch_energy(const double *eterm, int n, double *out, int precision, hipStream_t s);
 
 namespace {
@@ -355,8 +356,8 @@ __global__ void k_tree_root(const uint64_t *__restrict__ keys, int n, const int 
 
 __global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ keys, int n,
                                                    const int *__restrict__ isrep, const int *__restrict__ nodeid,
-                                                   BHNode *__restrict__ nodes, int *__restrict__ misc,
-                                                   const int *__restrict__ status) {
+                                                   BHNode *__restrict__ nodes, int *__restrict__ npts,
+                                                   int *__restrict__ misc, const int *__restrict__ status) {
     const int i = blockIdx.x * WG + threadIdx.x;
     if (*status) return;
     if (i >= n - 1 || !isrep[i]) return;
@@ -389,6 +390,7 @@ __global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ 
     nodes[v].depth = d;
     nodes[v].nint = nint;
     nodes[v].cnt = 0;
+    npts[v] = R - L + 1;
     if (v == misc[1]) nodes[v].parent = -1;
 }
 
@@ -418,17 +420,61 @@ __device__ __forceinline__ void centroid_of(BHNode *nodes, const double2 *spos, 
     nodes[v].c = make_double2(sx, sy);
 }
 
-__global__ __launch_bounds__(WG) void k_centroid(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
-                                                 const int *__restrict__ misc) {
+// Centroids bottom-up in two launches that keep the agent-scope handoffs to the top of the
+// tree.  A "small" node holds at most SMALL_CELL points.  k_centroid_small: one thread per
+// small subtree (a small node whose parent is not small) computes its nodes in post-order,
+// with no synchronisation.  k_centroid_top: the large nodes, from those without large
+// children upwards; the last large child to arrive at a parent (agent-scope acq_rel counter)
+// computes it; small children were done by the previous launch.  Every centroid is formed by
+// centroid_of from its children, so the order of the operations is the reference's.
+constexpr int SMALL_CELL = 32;
+constexpr int SUB_STACK = 64;  // a small subtree has <= 31 internal nodes, each pushed twice
+
+__global__ __launch_bounds__(64) void k_centroid_small(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                       const int *__restrict__ npts, const int *__restrict__ misc) {
+    __shared__ int stk[64][SUB_STACK];
+    const int v = blockIdx.x * 64 + threadIdx.x;
+    if (misc[0] || v >= misc[2]) return;
+    const int par = nodes[v].parent;
+    if (npts[v] > SMALL_CELL || (par >= 0 && npts[par] <= SMALL_CELL)) return;
+    int *st = stk[threadIdx.x];
+    int sp = 0;
+    st[sp++] = v;
+    while (sp > 0) {
+        const int u = st[--sp];
+        if (u >= 0) {
+            st[sp++] = ~u;  // revisit after the children
+            const int4 ch = nodes[u].ch;
+            const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+            for (int q = 3; q >= 0; --q)
+                if (c[q] >= 0) st[sp++] = c[q];
+        } else {
+            centroid_of(nodes, spos, ~u);
+        }
+    }
+}
+
+__device__ __forceinline__ int large_children(const BHNode *nodes, const int *npts, int v) {
+    const int4 ch = nodes[v].ch;
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    int k = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && npts[c[q]] > SMALL_CELL) ? 1 : 0;
+    return k;
+}
+
+__global__ __launch_bounds__(WG) void k_centroid_top(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                     const int *__restrict__ npts, const int *__restrict__ misc) {
     int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
-    if (nodes[v].nint != 0) return;
+    if (npts[v] <= SMALL_CELL || large_children(nodes, npts, v) != 0) return;
     for (;;) {
         centroid_of(nodes, spos, v);
         const int u = nodes[v].parent;
         if (u < 0) break;
         const int old = __hip_atomic_fetch_add(&nodes[u].cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != nodes[u].nint - 1) break;
+        if (old != large_children(nodes, npts, u) - 1) break;
         v = u;
     }
 }
@@ -822,21 +868,25 @@ __global__ __launch_bounds__(WALK_T) void k_walk(const int *__restrict__ rowptr,
 }
 
 // The one-workgroup walk with the batch's first entries prefetched (batches of at most
-// WALK_MAXB vertices).  Besides the staged rowptr slice, own positions and repulsion, each
-// thread loads during batch b the column ids, weights and neighbour positions of its first
-// WALK_PF entries of batch b+1.  A neighbour in batch b is moved after that load: at the
-// start of batch b+1 such entries take the moved position from the LDS copy the move phase
-// keeps (newpos), every other neighbour's position is final for the iteration.  A batch
-// whose entries fit in WALK_PF per thread thus reads no memory on its critical path.
-// Same sums, in the same order, as k_walk.
+// WALK_MAXB vertices).  Each thread owns a contiguous run of the batch's entries; the first
+// WALK_PF of them are loaded ahead in two stages: column ids and weights two batches ahead,
+// neighbour positions one batch ahead (both issued when a batch starts, so a whole batch
+// step hides their latency).  A neighbour moved after its position was loaded belongs to
+// the batch just finished: it is patched from the LDS copy of that batch's moves (newpos);
+// every other neighbour's position is final for the iteration when it is loaded.  The rowptr
+// slice, own positions and repulsion of the next batch are staged one batch ahead.  A batch
+// whose runs fit in WALK_PF entries per thread reads no memory on its critical path.  Same
+// sums, in the same order, as k_walk with WALK_PFT threads.
 constexpr int WALK_PF = 4;
+constexpr int WALK_PFT = 512;
 
-__global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowptr, const int *__restrict__ colids,
-                                                     const float *__restrict__ w_edge, double W,
-                                                     double2 *__restrict__ pos, const double2 *__restrict__ rep,
-                                                     int n, int B, double step, double *__restrict__ eterm,
-                                                     const int *__restrict__ status) {
-    constexpr int T = WALK_T;
+__global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ rowptr,
+                                                       const int *__restrict__ colids,
+                                                       const float *__restrict__ w_edge, double W,
+                                                       double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                       int n, int B, double step, double *__restrict__ eterm,
+                                                       const int *__restrict__ status) {
+    constexpr int T = WALK_PFT;
     constexpr int PER = (WALK_MAXB + T - 1) / T;  // staged vertices per thread
     __shared__ double2 head[T], tail[T];
     __shared__ int hrow[T], trow[T];
@@ -847,9 +897,10 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
     const int nb = (n + B - 1) / B;
     int nrp[PER + 1];
     double2 npos[PER], nrep[PER];
-    int pcid[WALK_PF];
-    double2 ppos[WALK_PF];
-    double pw[WALK_PF];
+    // prefetch stages: 0 = this batch, 1 = next batch, 2 = the one after
+    int pc0[WALK_PF], pc1[WALK_PF], pc2[WALK_PF];
+    float pw0[WALK_PF], pw1[WALK_PF], pw2[WALK_PF];
+    double2 pp0[WALK_PF], pp1[WALK_PF];
     auto fetch = [&](int b) {  // staged data of batch b
         const int b0 = b * B, bc = min(n, b0 + B) - b0;
 #pragma unroll
@@ -864,22 +915,26 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
             nrep[j] = k < bc ? rep[b0 + k] : make_double2(0.0, 0.0);
         }
     };
-    auto prefetch = [&](int b) {  // first entries of this thread's share of batch b
+    auto load_ids = [&](int b, int (&pc)[WALK_PF], float (&pw)[WALK_PF]) {  // first entries of batch b
         const int b0 = b * B, b1 = min(n, b0 + B);
         const int E0 = __builtin_amdgcn_readfirstlane(rowptr[b0]);
         const int E1 = __builtin_amdgcn_readfirstlane(rowptr[b1]);
         const int c = max(1, (E1 - E0 + T - 1) / T);
         const int s = E0 + t * c, e = min(E1, s + c);
 #pragma unroll
-        for (int g = 0; g < WALK_PF; ++g) pcid[g] = s + g < e ? colids[s + g] : -1;
-#pragma unroll
         for (int g = 0; g < WALK_PF; ++g) {
-            ppos[g] = pcid[g] >= 0 ? pos[pcid[g]] : make_double2(0.0, 0.0);
-            pw[g] = (w_edge && s + g < e) ? W * (double)w_edge[s + g] : W;
+            pc[g] = s + g < e ? colids[s + g] : -1;
+            pw[g] = (w_edge && s + g < e) ? w_edge[s + g] : 1.0f;
         }
     };
+    auto load_pos = [&](const int (&pc)[WALK_PF], double2 (&pp)[WALK_PF]) {
+#pragma unroll
+        for (int g = 0; g < WALK_PF; ++g) pp[g] = pc[g] >= 0 ? pos[pc[g]] : make_double2(0.0, 0.0);
+    };
     fetch(0);
-    prefetch(0);
+    load_ids(0, pc0, pw0);
+    load_pos(pc0, pp0);
+    if (nb > 1) load_ids(1, pc1, pw1);
     int pb0 = 0, pb1 = 0;  // the previous batch, whose moved positions are in newpos
     for (int b = 0; b < nb; ++b) {
         const int b0 = b * B, b1 = min(n, b0 + B), bc = b1 - b0;
@@ -893,11 +948,20 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
             const int k = t + j * T;
             if (k < bc) bpos[k] = npos[j];
         }
-        // prefetched neighbours that the previous batch moved
+        // this batch's prefetched neighbours that the previous batch moved
 #pragma unroll
         for (int g = 0; g < WALK_PF; ++g)
-            if (pcid[g] >= pb0 && pcid[g] < pb1) ppos[g] = newpos[pcid[g] - pb0];
+            if (pc0[g] >= pb0 && pc0[g] < pb1) pp0[g] = newpos[pc0[g] - pb0];
         __syncthreads();
+        double2 cur_rep[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) cur_rep[j] = nrep[j];
+        // the loads for the next batches, issued before this batch's work
+        if (b + 1 < nb) {
+            load_pos(pc1, pp1);
+            fetch(b + 1);
+        }
+        if (b + 2 < nb) load_ids(b + 2, pc2, pw2);
         const int E0 = rps[0], E1 = rps[bc];
         const int total = E1 - E0;
         const int c = max(1, (total + T - 1) / T);
@@ -922,9 +986,9 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
                 if (k0 == s) {
 #pragma unroll
                     for (int g = 0; g < WALK_PF; ++g) {
-                        cid[g] = pcid[g];
-                        pj[g] = ppos[g];
-                        wk[g] = pw[g];
+                        cid[g] = pc0[g];
+                        pj[g] = pp0[g];
+                        wk[g] = w_edge ? W * (double)pw0[g] : W;
                     }
                 } else {
 #pragma unroll
@@ -966,14 +1030,6 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
             }
         }
         __syncthreads();
-        // the next batch's staged data and first entries, issued before the moves
-        double2 cur_rep[PER];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) cur_rep[j] = nrep[j];
-        if (b + 1 < nb) {
-            fetch(b + 1);
-            prefetch(b + 1);
-        }
         for (int i = b0 + t, j = 0; i < b1; i += T, ++j) {
             const int rs = rps[i - b0], re = rps[i + 1 - b0];
             double2 f = make_double2(0.0, 0.0);
@@ -1006,6 +1062,14 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
         __syncthreads();
         pb0 = b0;
         pb1 = b1;
+#pragma unroll
+        for (int g = 0; g < WALK_PF; ++g) {
+            pc0[g] = pc1[g];
+            pw0[g] = pw1[g];
+            pp0[g] = pp1[g];
+            pc1[g] = pc2[g];
+            pw1[g] = pw2[g];
+        }
     }
 }
 
@@ -1275,7 +1339,7 @@ int ensure_bh(BHWork &w, int n) {
         w.hist.ensure(4 * (size_t)256 * nblk) || w.scan_tmp.ensure(4 * (size_t)(256 * nblk + 1)) ||
         w.spos.ensure(16 * N) || w.frame.ensure(sizeof(double4) * 1024 + 64) || w.isrep.ensure(4 * N) ||
         w.nodeid.ensure(4 * N) || w.node_child.ensure(sizeof(BHNode) * N) || w.rep.ensure(16 * N) ||
-        w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_cxy.ensure(16 * N))
+        w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_cxy.ensure(16 * N) || w.node_cnt.ensure(4 * N))
         return BL_E_NOMEM;
     w.n_alloc = n;
     return BL_OK;
@@ -1321,9 +1385,11 @@ int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *
         if (rc) return rc;
         const int nsb = (n - 1 + SCAN_TILE - 1) / SCAN_TILE;
         hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, s, ka, n, nodeid, bsum + nsb, misc);
-        hipLaunchKernelGGL(k_tree_nodes, dim3(g1), dim3(WG), 0, s, ka, n, isrep, nodeid, nodes, misc, misc);
+        int *npts = w.node_cnt.as<int>();
+        hipLaunchKernelGGL(k_tree_nodes, dim3(g1), dim3(WG), 0, s, ka, n, isrep, nodeid, nodes, npts, misc, misc);
         // n-1 boundaries give at most n-1 nodes; threads beyond the node count (misc[2]) exit
-        hipLaunchKernelGGL(k_centroid, dim3(g1), dim3(WG), 0, s, nodes, spos, misc);
+        hipLaunchKernelGGL(k_centroid_small, dim3((n + 63) / 64), dim3(64), 0, s, nodes, spos, npts, misc);
+        hipLaunchKernelGGL(k_centroid_top, dim3(g1), dim3(WG), 0, s, nodes, spos, npts, misc);
     } else {
         const int neg = -1;
         BL_HIP(hipMemcpyAsync(misc + 1, &neg, sizeof(int), hipMemcpyHostToDevice, s));
@@ -1409,7 +1475,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     const int n = g->n;
     hipStream_t s = g->stream;
     double2 *pos = reinterpret_cast<double2 *>(d_xy);
-    if (g->eterm.ensure(sizeof(double) * (size_t)n) || g->energy.ensure(sizeof(double) * 4) ||
+    if (g->eterm.ensure(sizeof(double) * (size_t)n) || g->energy.ensure(sizeof(double) * (4 + 256)) ||
         g->scratch.ensure(sizeof(double2) * (size_t)n))
         return BL_E_NOMEM;
     int rc = ensure_bh(g->bh, n);
@@ -1451,7 +1517,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK)
             ;
         else if (p->batch <= WALK_MAXB)
-            hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
+            hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_PFT), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(), n,
                                p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
         else

@@ -11,8 +11,8 @@
 // b = -2 S_uw with S the cell's normalised second moments, from 1/conj(w + delta) expanded
 // in delta) or the output of the learned model of mlp.h scaled by (side/rho)^2.
 //
-//   k_far_moments  count, mean, central second moments and first Morton rank of every cell,
-//                  bottom-up over the tree of bh.hip (last child to arrive computes the parent)
+//   k_far_small / k_far_top  count, mean, central second moments and first Morton rank of
+//                  every cell, bottom-up over the tree of bh.hip
 //   k_far_trav     traversal, one wave per 64 Morton-consecutive queries with a shared stack
 //                  of (node, lane mask); a cell is accepted when side^2 < theta^2 |c - v|^2.
 //                  The learned mode queues accepted cells per wave in LDS and evaluates the
@@ -95,18 +95,59 @@ __device__ void far_summary(const BHNode *nodes, const double2 *spos, FarCell *c
     cells[v] = out;
 }
 
-__global__ __launch_bounds__(WG) void k_far_moments(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
-                                                    const int *__restrict__ misc, FarCell *__restrict__ cells,
-                                                    int *__restrict__ arrive) {
+// The summaries bottom-up in two launches (as the centroids of bh.hip): one thread per
+// small subtree (at most FAR_SMALL points) in post-order, then the large nodes, where the
+// last large child to arrive at a parent (agent-scope acq_rel counter) computes it.
+constexpr int FAR_SMALL = 32;
+constexpr int FAR_STACK = 64;
+
+__global__ __launch_bounds__(64) void k_far_small(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                  const int *__restrict__ npts, const int *__restrict__ misc,
+                                                  FarCell *__restrict__ cells) {
+    __shared__ int stk[64][FAR_STACK];
+    const int v = blockIdx.x * 64 + threadIdx.x;
+    if (misc[0] || v >= misc[2]) return;
+    const int par = nodes[v].parent;
+    if (npts[v] > FAR_SMALL || (par >= 0 && npts[par] <= FAR_SMALL)) return;
+    int *st = stk[threadIdx.x];
+    int sp = 0;
+    st[sp++] = v;
+    while (sp > 0) {
+        const int u = st[--sp];
+        if (u >= 0) {
+            st[sp++] = ~u;
+            const int4 ch = nodes[u].ch;
+            const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+            for (int q = 3; q >= 0; --q)
+                if (c[q] >= 0) st[sp++] = c[q];
+        } else {
+            far_summary(nodes, spos, cells, ~u);
+        }
+    }
+}
+
+__device__ __forceinline__ int far_large_children(const BHNode *nodes, const int *npts, int v) {
+    const int4 ch = nodes[v].ch;
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    int k = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && npts[c[q]] > FAR_SMALL) ? 1 : 0;
+    return k;
+}
+
+__global__ __launch_bounds__(WG) void k_far_top(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                const int *__restrict__ npts, const int *__restrict__ misc,
+                                                FarCell *__restrict__ cells, int *__restrict__ arrive) {
     int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
-    if (nodes[v].nint != 0) return;
+    if (npts[v] <= FAR_SMALL || far_large_children(nodes, npts, v) != 0) return;
     for (;;) {
         far_summary(nodes, spos, cells, v);
         const int u = nodes[v].parent;
         if (u < 0) break;
         const int old = __hip_atomic_fetch_add(&arrive[u], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != nodes[u].nint - 1) break;
+        if (old != far_large_children(nodes, npts, u) - 1) break;
         v = u;
     }
 }
@@ -419,8 +460,11 @@ int far_cells_of(BHWork &w, int n, hipStream_t s) {
     if (rc) return rc;
     if (n >= 2) {
         BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
-        hipLaunchKernelGGL(k_far_moments, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
-                           w.spos.as<double2>(), w.misc.as<int>(), w.far_cells.as<FarCell>(), w.far_arrive.as<int>());
+        hipLaunchKernelGGL(k_far_small, dim3((n + 63) / 64), dim3(64), 0, s, w.node_child.as<BHNode>(),
+                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>());
+        hipLaunchKernelGGL(k_far_top, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
+                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>(),
+                           w.far_arrive.as<int>());
         BL_HIP(hipGetLastError());
     }
     return BL_OK;

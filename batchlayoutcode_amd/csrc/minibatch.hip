@@ -424,17 +424,33 @@ IterPlan plan_iter(int n, int bmax) {
 
 // ---- energy reductions (shared with the Barnes-Hut path) -------------------------------
 // Fixed-order tree sum of |F_i|^2 (FP32 mode).
-__global__ __launch_bounds__(1024) void k_energy_tree(const double *__restrict__ eterm, int n, double *out) {
-    __shared__ double s[1024];
-    double a = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) a += eterm[i];
-    s[threadIdx.x] = a;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
+// Energy sum_i |F_i|^2 in two fixed-order stages: ENERGY_PARTS workgroups each add a
+// contiguous share (threads strided, then a wave and workgroup reduction), then one wave adds
+// the shares in order.  Deterministic; out[0] = the sum, out[4 ..] = the shares.
+constexpr int ENERGY_PARTS = 256;
+
+__global__ __launch_bounds__(256) void k_energy_part(const double *__restrict__ eterm, int n, double *out) {
+    __shared__ double red[4];
+    const int ch = (n + ENERGY_PARTS - 1) / ENERGY_PARTS;
+    const int i0 = blockIdx.x * ch, i1 = min(n, i0 + ch);
+    double a = 0.0, b = 0.0;
+    int i = i0 + (int)threadIdx.x;
+    for (; i + 256 < i1; i += 512) {
+        a += eterm[i];
+        b += eterm[i + 256];
     }
-    if (threadIdx.x == 0) *out = s[0];
+    if (i < i1) a += eterm[i];
+    a = wave_sum(a + b);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) out[4 + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ __launch_bounds__(64) void k_energy_final(double *out) {
+    double a = 0.0;
+    for (int k = threadIdx.x; k < ENERGY_PARTS; k += 64) a += out[4 + k];
+    a = wave_sum(a);
+    if (threadIdx.x == 0) out[0] = a;
 }
 
 // Sequential sum in vertex order, ENERGY += |F_i|^2 as the reference (FP64 mode).
@@ -445,11 +461,14 @@ __global__ void k_energy_seq(const double *__restrict__ eterm, int n, double *ou
     *out = e;
 }
 
+// out: 4 + ENERGY_PARTS doubles (the sum in out[0])
 int launch_energy(const double *eterm, int n, double *out, int precision, hipStream_t s) {
     if (precision == BL_PREC_FP64)
         hipLaunchKernelGGL(k_energy_seq, dim3(1), dim3(64), 0, s, eterm, n, out);
-    else
-        hipLaunchKernelGGL(k_energy_tree, dim3(1), dim3(1024), 0, s, eterm, n, out);
+    else {
+        hipLaunchKernelGGL(k_energy_part, dim3(ENERGY_PARTS), dim3(256), 0, s, eterm, n, out);
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, s, out);
+    }
     BL_HIP(hipGetLastError());
     return BL_OK;
 }
@@ -494,7 +513,7 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     }
     if (g->part.ensure(sizeof(double2) * part_elems)) return BL_E_NOMEM;
     if (g->eterm.ensure(sizeof(double) * (size_t)n)) return BL_E_NOMEM;
-    if (g->energy.ensure(sizeof(double) * 4)) return BL_E_NOMEM;
+    if (g->energy.ensure(sizeof(double) * (4 + ENERGY_PARTS))) return BL_E_NOMEM;
     const size_t ctr_bytes = sizeof(unsigned) * 3 * NSTRIPE * CSTRIDE + 256;
     if (pl.NW > 0 && g->scratch.ensure(std::max(ctr_bytes, g->scratch.bytes))) return BL_E_NOMEM;
     double2 *part = g->part.as<double2>();
