@@ -868,25 +868,21 @@ __global__ __launch_bounds__(WALK_T) void k_walk(const int *__restrict__ rowptr,
 }
 
 // The one-workgroup walk with the batch's first entries prefetched (batches of at most
-// WALK_MAXB vertices).  Each thread owns a contiguous run of the batch's entries; the first
-// WALK_PF of them are loaded ahead in two stages: column ids and weights two batches ahead,
-// neighbour positions one batch ahead (both issued when a batch starts, so a whole batch
-// step hides their latency).  A neighbour moved after its position was loaded belongs to
-// the batch just finished: it is patched from the LDS copy of that batch's moves (newpos);
-// every other neighbour's position is final for the iteration when it is loaded.  The rowptr
-// slice, own positions and repulsion of the next batch are staged one batch ahead.  A batch
-// whose runs fit in WALK_PF entries per thread reads no memory on its critical path.  Same
-// sums, in the same order, as k_walk with WALK_PFT threads.
+// WALK_MAXB vertices).  Besides the staged rowptr slice, own positions and repulsion, each
+// thread loads during batch b the column ids, weights and neighbour positions of its first
+// WALK_PF entries of batch b+1.  A neighbour in batch b is moved after that load: at the
+// start of batch b+1 such entries take the moved position from the LDS copy the move phase
+// keeps (newpos), every other neighbour's position is final for the iteration.  A batch
+// whose entries fit in WALK_PF per thread thus reads no memory on its critical path.
+// Same sums, in the same order, as k_walk.
 constexpr int WALK_PF = 4;
-constexpr int WALK_PFT = 512;
 
-__global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ rowptr,
-                                                       const int *__restrict__ colids,
-                                                       const float *__restrict__ w_edge, double W,
-                                                       double2 *__restrict__ pos, const double2 *__restrict__ rep,
-                                                       int n, int B, double step, double *__restrict__ eterm,
-                                                       const int *__restrict__ status) {
-    constexpr int T = WALK_PFT;
+__global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowptr, const int *__restrict__ colids,
+                                                     const float *__restrict__ w_edge, double W,
+                                                     double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                     int n, int B, double step, double *__restrict__ eterm,
+                                                     const int *__restrict__ status) {
+    constexpr int T = WALK_T;
     constexpr int PER = (WALK_MAXB + T - 1) / T;  // staged vertices per thread
     __shared__ double2 head[T], tail[T];
     __shared__ int hrow[T], trow[T];
@@ -897,10 +893,9 @@ __global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ ro
     const int nb = (n + B - 1) / B;
     int nrp[PER + 1];
     double2 npos[PER], nrep[PER];
-    // prefetch stages: 0 = this batch, 1 = next batch, 2 = the one after
-    int pc0[WALK_PF], pc1[WALK_PF], pc2[WALK_PF];
-    float pw0[WALK_PF], pw1[WALK_PF], pw2[WALK_PF];
-    double2 pp0[WALK_PF], pp1[WALK_PF];
+    int pcid[WALK_PF];
+    double2 ppos[WALK_PF];
+    double pw[WALK_PF];
     auto fetch = [&](int b) {  // staged data of batch b
         const int b0 = b * B, bc = min(n, b0 + B) - b0;
 #pragma unroll
@@ -915,26 +910,22 @@ __global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ ro
             nrep[j] = k < bc ? rep[b0 + k] : make_double2(0.0, 0.0);
         }
     };
-    auto load_ids = [&](int b, int (&pc)[WALK_PF], float (&pw)[WALK_PF]) {  // first entries of batch b
+    auto prefetch = [&](int b) {  // first entries of this thread's share of batch b
         const int b0 = b * B, b1 = min(n, b0 + B);
         const int E0 = __builtin_amdgcn_readfirstlane(rowptr[b0]);
         const int E1 = __builtin_amdgcn_readfirstlane(rowptr[b1]);
         const int c = max(1, (E1 - E0 + T - 1) / T);
         const int s = E0 + t * c, e = min(E1, s + c);
 #pragma unroll
+        for (int g = 0; g < WALK_PF; ++g) pcid[g] = s + g < e ? colids[s + g] : -1;
+#pragma unroll
         for (int g = 0; g < WALK_PF; ++g) {
-            pc[g] = s + g < e ? colids[s + g] : -1;
-            pw[g] = (w_edge && s + g < e) ? w_edge[s + g] : 1.0f;
+            ppos[g] = pcid[g] >= 0 ? pos[pcid[g]] : make_double2(0.0, 0.0);
+            pw[g] = (w_edge && s + g < e) ? W * (double)w_edge[s + g] : W;
         }
     };
-    auto load_pos = [&](const int (&pc)[WALK_PF], double2 (&pp)[WALK_PF]) {
-#pragma unroll
-        for (int g = 0; g < WALK_PF; ++g) pp[g] = pc[g] >= 0 ? pos[pc[g]] : make_double2(0.0, 0.0);
-    };
     fetch(0);
-    load_ids(0, pc0, pw0);
-    load_pos(pc0, pp0);
-    if (nb > 1) load_ids(1, pc1, pw1);
+    prefetch(0);
     int pb0 = 0, pb1 = 0;  // the previous batch, whose moved positions are in newpos
     for (int b = 0; b < nb; ++b) {
         const int b0 = b * B, b1 = min(n, b0 + B), bc = b1 - b0;
@@ -948,20 +939,11 @@ __global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ ro
             const int k = t + j * T;
             if (k < bc) bpos[k] = npos[j];
         }
-        // this batch's prefetched neighbours that the previous batch moved
+        // prefetched neighbours that the previous batch moved
 #pragma unroll
         for (int g = 0; g < WALK_PF; ++g)
-            if (pc0[g] >= pb0 && pc0[g] < pb1) pp0[g] = newpos[pc0[g] - pb0];
+            if (pcid[g] >= pb0 && pcid[g] < pb1) ppos[g] = newpos[pcid[g] - pb0];
         __syncthreads();
-        double2 cur_rep[PER];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) cur_rep[j] = nrep[j];
-        // the loads for the next batches, issued before this batch's work
-        if (b + 1 < nb) {
-            load_pos(pc1, pp1);
-            fetch(b + 1);
-        }
-        if (b + 2 < nb) load_ids(b + 2, pc2, pw2);
         const int E0 = rps[0], E1 = rps[bc];
         const int total = E1 - E0;
         const int c = max(1, (total + T - 1) / T);
@@ -986,9 +968,9 @@ __global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ ro
                 if (k0 == s) {
 #pragma unroll
                     for (int g = 0; g < WALK_PF; ++g) {
-                        cid[g] = pc0[g];
-                        pj[g] = pp0[g];
-                        wk[g] = w_edge ? W * (double)pw0[g] : W;
+                        cid[g] = pcid[g];
+                        pj[g] = ppos[g];
+                        wk[g] = pw[g];
                     }
                 } else {
 #pragma unroll
@@ -1030,6 +1012,14 @@ __global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ ro
             }
         }
         __syncthreads();
+        // the next batch's staged data and first entries, issued before the moves
+        double2 cur_rep[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) cur_rep[j] = nrep[j];
+        if (b + 1 < nb) {
+            fetch(b + 1);
+            prefetch(b + 1);
+        }
         for (int i = b0 + t, j = 0; i < b1; i += T, ++j) {
             const int rs = rps[i - b0], re = rps[i + 1 - b0];
             double2 f = make_double2(0.0, 0.0);
@@ -1062,14 +1052,6 @@ __global__ __launch_bounds__(WALK_PFT) void k_walk_pf(const int *__restrict__ ro
         __syncthreads();
         pb0 = b0;
         pb1 = b1;
-#pragma unroll
-        for (int g = 0; g < WALK_PF; ++g) {
-            pc0[g] = pc1[g];
-            pw0[g] = pw1[g];
-            pp0[g] = pp1[g];
-            pc1[g] = pc2[g];
-            pw1[g] = pw2[g];
-        }
     }
 }
 
@@ -1517,7 +1499,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK)
             ;
         else if (p->batch <= WALK_MAXB)
-            hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_PFT), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
+            hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(), n,
                                p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
         else
