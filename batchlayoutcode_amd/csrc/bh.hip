@@ -220,6 +220,51 @@ __global__ __launch_bounds__(WG) void k_scan_apply(const int *__restrict__ in, i
     }
 }
 
+// Exclusive scan of up to SCAN_SINGLE_MAX ints by one workgroup: tiles of 1024 x 4, each
+// thread scans its 4 in order, the thread sums are scanned across the workgroup (waves by
+// shuffles, then the wave totals), and a carry runs from tile to tile.  *total = the sum.
+constexpr int SCAN_SINGLE_MAX = 32768;
+
+__global__ __launch_bounds__(1024) void k_scan_single(const int *__restrict__ in, int n, int *__restrict__ out,
+                                                      int *__restrict__ total) {
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += 4096) {
+        int v[4];
+        int a = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = base + 4 * t + k;
+            v[k] = i < n ? in[i] : 0;
+            a += v[k];
+        }
+        // inclusive scan of a over the wave
+        int x = a;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        int before = carry;
+        for (int q = 0; q < w; ++q) before += wsum[q];
+        int tile = 0;
+        for (int q = 0; q < 16; ++q) tile += wsum[q];
+        int run = before + x - a;  // exclusive prefix of this thread's first item
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = base + 4 * t + k;
+            if (i < n) out[i] = run;
+            run += v[k];
+        }
+        carry += tile;
+        __syncthreads();
+    }
+    if (t == 0) *total = carry;
+}
+
 // ---------------------------------------------------------------------------------------
 // 3. LSD radix sort, 8-bit digits.  hist is digit-major: hist[d * nblk + blk].
 __global__ __launch_bounds__(WG) void k_sort_hist(const uint64_t *__restrict__ keys, int n, int shift, int nblk,
@@ -1304,6 +1349,11 @@ struct TreeBufs {
 
 int scan_excl(const int *in, int n, int *out, int *bsum, hipStream_t s) {
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (n <= SCAN_SINGLE_MAX) {  // one launch instead of three
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, s, in, n, out, bsum + nb);
+        BL_HIP(hipGetLastError());
+        return BL_OK;
+    }
     hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(WG), 0, s, in, n, bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(WG), 0, s, bsum, nb);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(WG), 0, s, in, n, bsum, out);
