@@ -74,7 +74,15 @@ __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *r
                 const bool act = slot < bc;
                 const double2 pi = gp[b0 + (act ? slot : 0)];
                 double ax = 0.0, ay = 0.0;
-                for (int u = j0; u < j1; ++u) {
+                int u = j0;
+                for (; u + 8 <= j1; u += 8) {  // eight positions in flight per load batch
+                    double2 q[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) q[k] = gp[u + k];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) pair_rep<REFINE>(q[k].x, q[k].y, pi.x, pi.y, ax, ay);
+                }
+                for (; u < j1; ++u) {
                     const double2 q = gp[u];
                     pair_rep<REFINE>(q.x, q.y, pi.x, pi.y, ax, ay);
                 }
@@ -102,22 +110,23 @@ __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *r
                 gp[b0 + slot] = moved(gp[b0 + slot], f.x, f.y, step);
                 e2 += f.x * f.x + f.y * f.y;
             }
-            eb[tid] = e2;
+            // energy of the batch: wave sums, then the 16 wave sums in order (one barrier, which
+            // also makes the moves visible to the next batch)
+            e2 = wave_sum(e2);
+            if ((tid & 63) == 0) eb[tid >> 6] = e2;
             __syncthreads();
-            for (int w = BT / 2; w > 0; w >>= 1) {
-                if (tid < w) eb[tid] += eb[tid + w];
-                __syncthreads();
-            }
-            E += eb[0];  // batches added in order
-            __syncthreads();
+            double eb_sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < BT / 64; ++q) eb_sum += eb[q];
+            E += eb_sum;  // batches added in order
         }
         step = step * a.decay;
     }
     if (in_lds) {
         for (int k = tid; k < n; k += BT) a.pos[v0 + k] = P[k];
     }
-    // every thread holds the same E (batch sums come from eb[0]); all of them store it so that
-    // no per-lane branch sits next to the queue loop's back edge
+    // every thread holds the same E (the batch sums come from the same wave sums); all of them
+    // store it so that no per-lane branch sits next to the queue loop's back edge
     a.energies[g] = E;
     a.finish[g] = wall_clock64();
     __syncthreads();
@@ -127,7 +136,7 @@ __global__ __launch_bounds__(BT) void k_batched(BatchedArgs a) {
     extern __shared__ double2 P[];  // positions of the current graph when it fits
     __shared__ double2 red[BT];
     __shared__ double2 fb[MAXB];
-    __shared__ double eb[BT];
+    __shared__ double eb[BT / 64];
     __shared__ int slot;
     const int tid = threadIdx.x;
     if (tid == 0) a.wg_start[blockIdx.x] = wall_clock64();
@@ -174,7 +183,7 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
     BL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     BL_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-    const int static_lds = (int)(sizeof(double2) * (BT + MAXB) + sizeof(double) * BT + 64);
+    const int static_lds = (int)(sizeof(double2) * (BT + MAXB) + sizeof(double) * (BT / 64) + 64);
     int maxn = 0;
     for (int k = 0; k < count; ++k) maxn = std::max(maxn, voff[k + 1] - voff[k]);
     const int cap = std::max(0, std::min(maxn, (maxlds - static_lds) / (int)sizeof(double2)));

@@ -13,7 +13,7 @@ import numpy as np
 from . import api, graphs
 
 
-def training_cells(theta=1.0, seed=12, sizes=(140, 20_000, 14), layout_iters=15, queries=8000):
+def training_cells(theta=1.0, seed=12, sizes=(140, 20_000, 14), layout_iters=15, queries=3000):
     """Rows from three layouts: a grid (sizes[0]^2), a random geometric graph (sizes[1]) and an
     R-MAT graph (scale sizes[2])."""
     rng = np.random.default_rng(seed)
@@ -32,7 +32,7 @@ def training_cells(theta=1.0, seed=12, sizes=(140, 20_000, 14), layout_iters=15,
     return np.concatenate(feats), np.concatenate(targs)
 
 
-def train(theta=1.2, epochs=60, batch=4096, lr=2e-3, seed=7):
+def train(theta=1.0, epochs=12, batch=4096, lr=2e-3, seed=7):
     """Returns (parameters, info dict).  Cells are drawn at the opening threshold theta, so the
     model covers every threshold up to it."""
     t0 = time.time()

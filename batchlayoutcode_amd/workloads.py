@@ -64,7 +64,7 @@ def main(argv=None):
     elif a.name == "delaunay2m-far":
         from . import farmodel
 
-        model, info = farmodel.train(theta=1.2)
+        model, info = farmodel.train(theta=1.0)
         print(json.dumps(dict(workload=a.name, model_training=info)), flush=True)
         rp, ci = graphs.csr_from_edges(*graphs.delaunay_holes())
         n = len(rp) - 1

@@ -102,7 +102,7 @@ def far_leg(api, graphs, rp, ci, init, iters, theta=1.0):
     """Physically consistent Barnes-Hut on the same graph: exact repulsion, then monopole,
     analytic quadrupole and learned far field over a sweep of opening thresholds;
     vertex-updates/s and final energy vs the exact layout (the model is trained on cells
-    accepted at `theta`, the largest threshold of the sweep)."""
+    accepted at `theta`)."""
     from batchlayoutcode_amd import farmodel
 
     n = len(rp) - 1
@@ -110,7 +110,7 @@ def far_leg(api, graphs, rp, ci, init, iters, theta=1.0):
     log(f"far field: {info['training_cells']} training cells, loss {info['loss_first']:.3e} -> "
         f"{info['loss_last']:.3e} ({info['data_seconds'] + info['train_seconds']:.1f} s)")
     res = {}
-    thetas = (1.2, 1.0, 0.8, 0.6, 0.4)
+    thetas = (1.2, 1.0, 0.8, 0.6, 0.4)  # the model, trained at theta, is used above it too
     with api.Graph(rp, ci) as g:
         g.set_far_model(model)
 
@@ -252,7 +252,7 @@ def main():
                fp32_mode=dict(value=n * K / secs32, ms_per_step=secs32 * 1e3 / K, pair_kernel_us=pair32_ms * 1e3,
                               achieved_tflops=achieved32, frac_of_fp32_peak=achieved32 / PEAK_FP32_TFLOPS))
     if not args.no_far:
-        out["far_field"] = far_leg(api, graphs, rp, ci, init, iters=10, theta=1.2)
+        out["far_field"] = far_leg(api, graphs, rp, ci, init, iters=10, theta=1.0)
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         log(f"cpu baseline on {threads} threads ...")
