@@ -116,6 +116,8 @@ extern "C" int bl_graph_create(int32_t n, int64_t nnz, const int32_t *rowptr, co
     g->nnz = nnz;
     g->has_w = w_edge != nullptr;
     g->eff_max_deg = maxdeg;
+    g->h_rowptr.assign(rowptr, rowptr + n + 1);
+    for (int32_t i = 0; i < n; ++i) g->max_row = std::max(g->max_row, rowptr[i + 1] - rowptr[i]);
     auto fail = [&](int code) {
         bl_graph_destroy(g);
         return code;

@@ -1100,6 +1100,219 @@ __global__ __launch_bounds__(WALK_T) void k_walk_pf(const int *__restrict__ rowp
     }
 }
 
+// The one-workgroup walk with one thread per batch vertex, for batches of at most WROW_T
+// vertices and WROW_CAP CSR entries whose rows are all short (at most WROW_MAXDEG entries).
+// The workgroup is split into WROW_T computing threads and WROW_T loading threads, so that
+// the gathers of one batch overlap the arithmetic of the previous one (a wave that issues a
+// batch's worth of random gathers stalls for the whole issue, so the two must not share a
+// wave):
+//   computing thread t owns vertex b0 + t of every batch, sums its row itself in CSR order
+//   (the reference's order) from the LDS stage and moves it, into LDS rings;
+//   the loading threads, during batch b, write the moves of batch b-1 to memory and fill the
+//   other half of the stage with batch b+1: its CSR entries (column id, weight, neighbour
+//   position, spread over the threads by entry) and its vertices' rows, positions and
+//   repulsion.  Column ids run a batch ahead of the positions they address, CSR ranges two.
+// One barrier ends each batch.
+//
+// Memory holds the iteration-start position of every vertex of the current batch (Jacobi
+// within a batch): moves reach memory only in the next batch.  A gather issued during batch
+// b for batch b+1 may see the old position of a vertex of batch b-1 (stored meanwhile) or b
+// (moved meanwhile), so every neighbour in batch b-2 or b-1 of the batch being computed
+// takes its position from the ring of moved positions.  Older batches were stored, and the
+// stores completed, before a barrier that every gather has passed since.
+constexpr int WROW_T = 256;
+constexpr int WROW_CAP = 2560;   // staged CSR entries per batch
+constexpr int WROW_S = WROW_CAP / WROW_T;
+constexpr int WROW_MAXDEG = 64;  // longest row for which this walk is chosen
+
+constexpr size_t wrow_lds(bool has_w) {
+    return sizeof(double2) * (3 * WROW_T + 2 * WROW_CAP + 4 * WROW_T) + sizeof(double) * 2 * WROW_T +
+           sizeof(int) * (2 * WROW_CAP + 4 * WROW_T) + (has_w ? sizeof(float) * 2 * WROW_CAP : 0);
+}
+
+// Whether k_walk_row can take batches of B vertices of graph g (cached per batch size).
+bool row_walk_fits(bl_graph *g, int B) {
+    if (g->row_walk_batch != B) {
+        bool ok = B <= WROW_T && g->nnz > 0 && g->max_row <= WROW_MAXDEG && (int)g->h_rowptr.size() == g->n + 1;
+        for (int b0 = 0; ok && b0 < g->n; b0 += B)
+            ok = g->h_rowptr[std::min(g->n, b0 + B)] - g->h_rowptr[b0] <= WROW_CAP;
+        g->row_walk_batch = B;
+        g->row_walk_ok = ok;
+    }
+    return g->row_walk_ok;
+}
+
+template <bool HAS_W>
+__global__ __launch_bounds__(2 * WROW_T) void k_walk_row(const int *__restrict__ rowptr, const int *__restrict__ colids,
+                                                         const float *__restrict__ w_edge, double W,
+                                                         double2 *__restrict__ pos, const double2 *__restrict__ rep,
+                                                         int n, int B, double step, double *__restrict__ eterm,
+                                                         const int *__restrict__ status) {
+    constexpr int T = WROW_T, S = WROW_S, C = WROW_CAP;
+    extern __shared__ double2 lds[];
+    double2 *ring = lds;                    // [3][T] moved positions of batch c (slot c % 3)
+    double2 *spos = ring + 3 * T;           // [2][C] staged neighbour positions (half c & 1)
+    double2 *sv = spos + 2 * C;             // [2][T] staged own positions
+    double2 *sr = sv + 2 * T;               // [2][T] staged repulsion
+    double *ering = reinterpret_cast<double *>(sr + 2 * T);  // [2][T] energy terms of batch c
+    int *scid = reinterpret_cast<int *>(ering + 2 * T);      // [2][C] staged column ids
+    int *srs = scid + 2 * C;                                 // [2][T] staged row starts
+    int *sre = srs + 2 * T;                                  // [2][T] staged row ends
+    float *sw = reinterpret_cast<float *>(sre + 2 * T);      // [2][C] staged weights (HAS_W)
+    if (__builtin_amdgcn_readfirstlane(*status)) return;
+    const int nb = (n + B - 1) / B;
+    if (threadIdx.x < T) {  // ---- computing threads ----
+        const int t = threadIdx.x;
+        __syncthreads();  // batch 0 staged
+        for (int b = 0; b < nb; ++b) {
+            const int b0 = b * B, b1 = min(n, b0 + B);
+            const int m1 = b0 - B, m2 = b0 - 2 * B;  // first vertices of batches b-1 and b-2
+            const double2 *ring1 = ring + ((b + 2) % 3) * T, *ring2 = ring + ((b + 1) % 3) * T;
+            const int cur = b & 1;
+            const int i = b0 + t;
+            if (i < b1) {
+                const int E0 = srs[cur * T];
+                const int rs = srs[cur * T + t], re = sre[cur * T + t];
+                const double2 v = sv[cur * T + t], r = sr[cur * T + t];
+                double ax = 0.0, ay = 0.0;
+                for (int e = rs; e < re; e += 4) {
+                    int j[4];
+                    double2 pj[4];
+                    double wk[4];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int k = cur * C + min(e + g, re - 1) - E0;
+                        j[g] = scid[k];
+                        pj[g] = spos[k];
+                        wk[g] = HAS_W ? W * (double)sw[k] : W;
+                    }
+                    double dx[4], dy[4], sc[4];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        if (j[g] >= m2 && j[g] < b0) pj[g] = j[g] >= m1 ? ring1[j[g] - m1] : ring2[j[g] - m2];
+                        dx[g] = pj[g].x - v.x;
+                        dy[g] = pj[g].y - v.y;
+                        sc[g] = wk[g] * sqrt(dx[g] * dx[g] + dy[g] * dy[g]);
+                    }
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        if (e + g >= re) break;
+                        ax = fma(dx[g], sc[g], ax);
+                        ay = fma(dy[g], sc[g], ay);
+                    }
+                }
+                const double2 f = make_double2(ax - r.x, ay - r.y);  // f - calcRepForce(x_i), :1059
+                const double mag = sqrt(f.x * f.x + f.y * f.y);
+                double2 q = v;
+                q.x = q.x + (f.x / mag) * step;
+                q.y = q.y + (f.y / mag) * step;
+                ring[(b % 3) * T + t] = q;
+                ering[cur * T + t] = f.x * f.x + f.y * f.y;
+            }
+            __syncthreads();  // end of batch b
+        }
+        return;
+    }
+    // ---- loading threads ----
+    const int t = threadIdx.x - T;
+    // CSR range of batch b (past the last batch: the empty range at nnz), loaded as vector loads
+    // into values equal in all lanes: a uniform load would become a scalar load, whose wait
+    // counter LDS accesses share.
+    auto bounds = [&](int b, int &a, int &e) {
+        const int ia = min(b * B, n), ie = min(n, b * B + B);
+        int va, ve;
+        __asm__("v_mov_b32 %0, %1" : "=v"(va) : "s"(ia));
+        __asm__("v_mov_b32 %0, %1" : "=v"(ve) : "s"(ie));
+        a = rowptr[va];
+        e = rowptr[ve];
+    };
+    // Every load below is unconditional, with its index clamped into the array: slots past a
+    // batch's entries (or vertices past its end) hold some valid entry and are never read.
+    auto load_cids = [&](int a, int e, int *c, float *w) {  // this thread's entries of a batch
+        a = __builtin_amdgcn_readfirstlane(a);
+        e = __builtin_amdgcn_readfirstlane(e);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int k = max(0, min(a + t + s * T, e - 1));
+            c[s] = colids[k];
+            if (HAS_W) w[s] = w_edge[k];
+        }
+    };
+    int cA[S], cB[S];  // column ids of this thread's entries of batches b+1 and b+2
+    float wA[S], wB[S];
+    double pAx[S], pAy[S];  // neighbour positions of the entries of batch b+1
+    int E1a, E1e, E2a, E2e;  // CSR ranges of batches b+1, b+2
+    // stage batch `half`: entries (column ids c/w, positions pAx/pAy) and own rows of batch bb
+    auto stage = [&](int bb, int half) {
+        const int i = min(bb * B + t, n - 1);
+        const int rs = rowptr[i], re = rowptr[i + 1];
+        const double2 v = pos[i], r = rep[i];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int k = half * C + t + s * T;
+            spos[k] = make_double2(pAx[s], pAy[s]);
+            scid[k] = cA[s];
+            if (HAS_W) sw[k] = wA[s];
+        }
+        srs[half * T + t] = rs;
+        sre[half * T + t] = re;
+        sv[half * T + t] = v;
+        sr[half * T + t] = r;
+    };
+    {
+        int E0a, E0e;
+        bounds(0, E0a, E0e);
+        bounds(1, E1a, E1e);
+        bounds(2, E2a, E2e);
+        load_cids(E0a, E0e, cA, wA);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double2 q = pos[cA[s]];
+            pAx[s] = q.x;
+            pAy[s] = q.y;
+        }
+        stage(0, 0);
+        load_cids(E1a, E1e, cA, wA);
+    }
+    __syncthreads();  // batch 0 staged
+    for (int b = 0; b < nb; ++b) {
+        if (b > 0) {  // the moves of batch b-1 to memory
+            const int c0 = (b - 1) * B;
+            if (t < B) {
+                pos[c0 + t] = ring[((b + 2) % 3) * T + t];
+                eterm[c0 + t] = ering[((b + 1) & 1) * T + t];
+            }
+        }
+        // batch b+1: positions of its entries; batch b+2: column ids; batch b+3: CSR range
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const double2 q = pos[cA[s]];
+            pAx[s] = q.x;
+            pAy[s] = q.y;
+        }
+        load_cids(E2a, E2e, cB, wB);
+        int E3a, E3e;
+        bounds(b + 3, E3a, E3e);
+        stage(b + 1, (b + 1) & 1);
+        // the stores of batch b-1 are complete before the barrier (gathers after it read them)
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // end of batch b
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            cA[s] = cB[s];
+            if (HAS_W) wA[s] = wB[s];
+        }
+        E1a = E2a; E1e = E2e;
+        E2a = E3a; E2e = E3e;
+    }
+    // the moves of the last batch
+    const int c0 = (nb - 1) * B;
+    if (t < B && c0 + t < n) {
+        pos[c0 + t] = ring[((nb - 1) % 3) * T + t];
+        eterm[c0 + t] = ering[((nb - 1) & 1) * T + t];
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // 8b. the batch walk over GW_G workgroups, for batches with many CSR entries.  One
 // workgroup keeps only a few hundred misses in flight, so a batch of tens of thousands of
@@ -1517,13 +1730,23 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     prof.s = s;
     double STEP = step0;
     // the batch walk over several workgroups pays off for batches with many entries;
-    // BL_WALK=grid|single overrides the choice
+    // BL_WALK=grid|single|pf|row overrides the choice (single: the best one-workgroup walk)
     const int nbt = (n + p->batch - 1) / p->batch;
     bool use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB &&
                     (double)g->nnz / (double)nbt >= (double)GW_MIN_ENTRIES;
+    bool use_row = p->precision != BL_PREC_FP64 && row_walk_fits(g, p->batch);
     if (const char *wenv = getenv("BL_WALK")) {
         if (!strcmp(wenv, "grid")) use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB;
         else if (!strcmp(wenv, "single")) use_grid = false;
+        else if (!strcmp(wenv, "pf")) use_grid = false, use_row = false;
+        else if (!strcmp(wenv, "row")) use_grid = false;
+    }
+    if (use_grid) use_row = false;
+    if (use_row) {
+        const size_t lb = wrow_lds(g->has_w);
+        BL_HIP(hipFuncSetAttribute(g->has_w ? reinterpret_cast<const void *>(k_walk_row<true>)
+                                            : reinterpret_cast<const void *>(k_walk_row<false>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
     }
     g->bh.gwalk_used = 0;
     // the exact mode builds no tree, so nothing else clears the status word the walk reads
@@ -1548,6 +1771,11 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->bh.misc.as<int>());
         else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK)
             ;
+        else if (use_row)
+            hipLaunchKernelGGL(g->has_w ? k_walk_row<true> : k_walk_row<false>, dim3(1), dim3(2 * WROW_T),
+                               wrow_lds(g->has_w), s, g->rowptr.as<int>(), g->colids.as<int>(),
+                               g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
+                               n, p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
         else if (p->batch <= WALK_MAXB)
             hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(), n,

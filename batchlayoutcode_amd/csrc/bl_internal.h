@@ -59,6 +59,10 @@ struct bl_graph {
     // CSR as given (algo 4 attraction walks every entry, algorithms.cpp:1055-1058)
     bl::DevBuf rowptr, colids, w_edge;
     bool has_w = false;
+    std::vector<int32_t> h_rowptr;  // host copy of rowptr (walk selection)
+    int32_t max_row = 0;            // longest CSR row
+    int32_t row_walk_batch = 0;     // batch size row_walk_ok was computed for
+    bool row_walk_ok = false;
     // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),
     // as a CSR of strictly increasing column ids per row (+ their weights)
     bl::DevBuf eff_rowptr, eff_colids, eff_w;

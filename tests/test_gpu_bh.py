@@ -154,3 +154,24 @@ def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo):
     d = diameter(out["single"])
     assert np.abs(out["grid"] - out["single"]).max() <= 1e-10 * d
     assert abs(out["grid_E"] - out["single_E"]) <= 1e-10 * out["single_E"]
+
+
+@pytest.mark.parametrize("weighted,batch,algo,prec", [(False, 256, 4, 0), (True, 256, 4, 0), (False, 100, 4, 0),
+                                                      (True, 37, 4, 0), (False, 256, 40, 0), (False, 256, 4, 2)])
+def test_row_walk_matches_entry_split_walk(bl, monkeypatch, weighted, batch, algo, prec):
+    # short rows and batches of at most 256 vertices take the one-thread-per-vertex walk; it
+    # must agree with the entry-split walk up to the order of partial sums of split rows
+    rp, ci = graphs.csr_from_edges(*graphs.rgg(20000, seed=5))
+    n = len(rp) - 1
+    w = np.random.default_rng(7).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
+    x0 = np.random.default_rng(8).random((n, 2))
+    print(f"[case] row walk weighted {weighted} batch {batch} algo {algo} precision {prec}", flush=True)
+    out = {}
+    for mode in ("single", "pf"):
+        monkeypatch.setenv("BL_WALK", mode)
+        with bl.Graph(rp, ci, w) as g:
+            out[mode], st = g.run(x0, algo=algo, iters=3, batch=batch, bht=1.2, precision=prec)
+            out[mode + "_E"] = st.energy
+    d = diameter(out["pf"])
+    assert np.abs(out["single"] - out["pf"]).max() <= 1e-10 * d
+    assert abs(out["single_E"] - out["pf_E"]) <= 1e-10 * out["pf_E"]
