@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <mutex>
 #include <new>
@@ -144,6 +145,24 @@ extern "C" int bl_graph_create(int32_t n, int64_t nnz, const int32_t *rowptr, co
                     hipMemcpyAsync(g->eff_w.p, ew.data(), sizeof(float) * ew.size(), hipMemcpyHostToDevice, s) !=
                         hipSuccess)))
         return fail(BL_E_DEVICE);
+    // correction pieces (one wave each in the all-pairs step)
+    g->h_cpre.assign((size_t)n + 1, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t np = std::max(1, (erp[i + 1] - erp[i] + CORR_PIECE - 1) / CORR_PIECE);
+        g->h_cpre[i + 1] = g->h_cpre[i] + np;
+        g->corr_max_pieces = std::max(g->corr_max_pieces, np);
+    }
+    {
+        std::vector<int32_t> pv((size_t)g->h_cpre[n]);
+        for (int32_t i = 0; i < n; ++i)
+            for (int32_t q = g->h_cpre[i]; q < g->h_cpre[i + 1]; ++q) pv[q] = i;
+        if (g->cpre.ensure(sizeof(int32_t) * (n + 1)) || g->cpv.ensure(sizeof(int32_t) * pv.size()))
+            return fail(BL_E_NOMEM);
+        if (hipMemcpyAsync(g->cpre.p, g->h_cpre.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(g->cpv.p, pv.data(), sizeof(int32_t) * pv.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+            return fail(BL_E_DEVICE);
+    }
     if (hipStreamSynchronize(s) != hipSuccess) return fail(BL_E_DEVICE);
     *out = g;
     return BL_OK;
@@ -153,7 +172,7 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
     if (!g) return BL_E_ARG;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
-                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model};
+                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model, &g->cpre, &g->cpv};
     for (DevBuf *b : all) b->release();
     g->bh.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);

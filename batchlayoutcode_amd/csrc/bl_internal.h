@@ -67,6 +67,11 @@ struct bl_graph {
     // as a CSR of strictly increasing column ids per row (+ their weights)
     bl::DevBuf eff_rowptr, eff_colids, eff_w;
     int32_t eff_max_deg = 0;
+    // neighbour-correction pieces of the all-pairs step: vertex i's merge-pointer neighbours
+    // in runs of at most CORR_PIECE, pieces [cpre[i], cpre[i+1]); cpv[p] is piece p's vertex
+    bl::DevBuf cpre, cpv;
+    std::vector<int32_t> h_cpre;
+    int32_t corr_max_pieces = 1;
     // workspace
     bl::DevBuf pos, part, eterm, energy, scratch;
     bl::BHWork bh;
@@ -155,6 +160,8 @@ struct Prof {
 // entry, which then acts as one neighbour (an empty row at the end meets nothing).
 // Appends to eci (column ids), ew (weights, when w is given) and sets erp[i+1] for each row
 // (erp[i0] must hold the running count already).
+constexpr int CORR_PIECE = 256;  // neighbour entries per correction wave
+
 inline void effective_neighbors(const int32_t *rowptr, const int32_t *colids, const float *w, int32_t i0, int32_t i1,
                                 int64_t end, std::vector<int32_t> &erp, std::vector<int32_t> &eci,
                                 std::vector<float> &ew, int32_t *max_deg) {

@@ -16,16 +16,18 @@
 // component with the FP64 difference).  The dense loop runs over all j; neighbours are not
 // branched on: a correction adds, in FP64, each neighbour's attraction and removes the
 // repulsion product the dense loop accumulated for it (the same operands, so it cancels).
-//   * k_iter32 (default): one persistent launch per iteration.  Worker workgroups own a j
-//     chunk held in LDS for the whole iteration and compute, batch after batch, the chunk
-//     partial of every batch vertex; reducer workgroups form each batch vertex's correction,
-//     sum its partials in a fixed order and move it.  Steps are ordered by striped atomic
-//     counters: a worker only waits for the previous batch's moves when its own chunk holds
-//     those vertices (and then only for that part of the chunk, which it processes last),
-//     so the reduction and move of batch b overlap the bulk of batch b+1.  Every wait has a
-//     deadline; on expiry the launch sets an error flag and all workgroups leave.
-//   * k_pairs32 + k_move32: two launches per batch step, used when the cooperative launch
-//     does not fit (very large n or batch).
+//   * k_pairs32 + k_move32 (default): two launches per batch step.  The pair kernel's
+//     chunk workgroups write each batch vertex's chunk partials; its correction waves take
+//     the neighbour corrections in pieces of at most CORR_PIECE entries (a hub's
+//     correction is spread over many waves); k_move32 adds them in a fixed order and moves.
+//   * k_iter32 (BL_PERSISTENT=1, mixed precision): one persistent launch per iteration.
+//     Worker workgroups own a j chunk held in LDS for the whole iteration and compute,
+//     batch after batch, the chunk partial of every batch vertex; reducer workgroups form
+//     each batch vertex's correction, sum its partials in a fixed order and move it.  Steps
+//     are ordered by striped atomic counters; every wait has a deadline, on expiry the
+//     launch sets an error flag and all workgroups leave.  Correct but slower than the
+//     per-step launches: each of its many agent-scope hand-offs writes back or invalidates
+//     an XCD's L2.
 //
 // FP64 mode (reference-grade): the reference's own operations, in its order.  The
 // j range is cut into the reference's 512-wide blocks (blocky, :717); each block's
@@ -244,32 +246,38 @@ __global__ __launch_bounds__(WG) void k_iter32(IterArgs a) {
 // Dense FP32 repulsion + neighbour correction (one launch per batch step).
 // grid = (G + ncorr, rows), 256 threads.  Workgroups x < G: batch slots [y*IPW, y*IPW+IPW)
 // against j chunk x, split over JS = 256/IPW j-subgroups added in LDS in subgroup order.
-// Workgroups x >= G (row 0 only): one wave per batch slot, neighbour correction.
-// part[slot * P + x] receives the chunk partial; part[slot * P + P-1] the correction.
+// Workgroups x >= G (row 0 only): one wave per correction piece of the batch (pieces
+// [pc0, pc0+npc): at most CORR_PIECE neighbour entries of one vertex, so a hub's correction
+// is spread over many waves instead of holding up the step).
+// part[slot * P + x] receives the chunk partial; part[slot * P + G + k] piece k's correction.
 template <int JS, bool REFINE>
 __global__ __launch_bounds__(WG) void k_pairs32(const double2 *__restrict__ pos, int n, int b0, int bc, int chunk,
                                                 int G, double2 *__restrict__ part, int P,
                                                 const int *__restrict__ erp, const int *__restrict__ eci,
-                                                const float *__restrict__ ew, double W, int law) {
+                                                const float *__restrict__ ew, double W, int law,
+                                                const int *__restrict__ cpre, const int *__restrict__ cpv, int pc0,
+                                                int npc) {
     constexpr int IPW = WG / JS;
     __shared__ double2 red[WG];
     const int tid = threadIdx.x;
     if ((int)blockIdx.x >= G) {
         if (blockIdx.y != 0) return;
-        const int slot = ((int)blockIdx.x - G) * (WG / 64) + tid / 64;
+        const int w = ((int)blockIdx.x - G) * (WG / 64) + tid / 64;
         const int lane = tid & 63;
-        if (slot >= bc) return;
-        const int i = b0 + slot;
+        if (w >= npc) return;
+        const int pc = pc0 + w;
+        const int i = cpv[pc], k = pc - cpre[i];
         const double2 pi = pos[i];
         double cx = 0.0, cy = 0.0;
-        const int kend = erp[i + 1];
-        for (int k = erp[i] + lane; k < kend; k += 64) {
-            const double2 pj = pos[eci[k]];
-            correct_pair<REFINE>(law, ew ? W * (double)ew[k] : W, pj.x - pi.x, pj.y - pi.y, cx, cy);
+        const int kbeg = erp[i] + k * CORR_PIECE;
+        const int kend = min(erp[i + 1], kbeg + CORR_PIECE);
+        for (int e = kbeg + lane; e < kend; e += 64) {
+            const double2 pj = pos[eci[e]];
+            correct_pair<REFINE>(law, ew ? W * (double)ew[e] : W, pj.x - pi.x, pj.y - pi.y, cx, cy);
         }
         cx = wave_sum(cx);
         cy = wave_sum(cy);
-        if (lane == 0) part[(size_t)slot * P + (P - 1)] = make_double2(cx, cy);
+        if (lane == 0) part[(size_t)(i - b0) * P + G + k] = make_double2(cx, cy);
         return;
     }
     const int li = tid % IPW;
@@ -345,17 +353,19 @@ __global__ __launch_bounds__(WG) void k_pairs64(const double2 *__restrict__ pos,
     part[(size_t)slot * P + blockIdx.x] = make_double2(fx, fy);
 }
 
-// FP32 fallback: one workgroup per batch vertex sums its P partials (FP64, fixed order:
-// strided per thread, then a fixed tree).
-__global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part, int P, double2 *__restrict__ pos,
-                                               int b0, int bc, double step, double *__restrict__ eterm) {
+// FP32 path: one workgroup per batch vertex sums its G chunk partials and its correction
+// pieces (FP64, fixed order: strided per thread, then a fixed tree).
+__global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part, int P, int G,
+                                               const int *__restrict__ cpre, double2 *__restrict__ pos, int b0,
+                                               int bc, double step, double *__restrict__ eterm) {
     __shared__ double2 red[WG / 64];
     const int slot = (int)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (slot >= bc) return;
     const double2 *row = part + (size_t)slot * P;
+    const int pe = G + (cpre[b0 + slot + 1] - cpre[b0 + slot]);
     double sx = 0.0, sy = 0.0;
-    for (int p = tid; p < P; p += WG) {
+    for (int p = tid; p < pe; p += WG) {
         const double2 v = row[p];
         sx += v.x;
         sy += v.y;
@@ -496,17 +506,19 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
         part_elems = (size_t)bmax * P;
     } else {
         // per-step geometry (also the fallback when the persistent launch is refused)
-        // 64 batch slots x 4 j-subgroups per workgroup: ~1024 workgroups (4 per CU) with
-        // about 256 chunk partials per batch vertex for the move kernel to add up
+        // 64 batch slots x 4 j-subgroups per workgroup: at least ~1024 workgroups (4 per CU),
+        // and j chunks of at most ~CHUNK_MAX positions, whose scalar loads the rows of a chunk
+        // then mostly share in the scalar cache
+        constexpr int CHUNK_MAX = 1024;
         JS = 4;
         const int ipw = WG / JS;
         const int rows = (bmax + ipw - 1) / ipw;
-        const int target = std::max(1, 1024 / rows);
+        const int target = std::max(std::max(1, 1024 / rows), (n + CHUNK_MAX - 1) / CHUNK_MAX);
         const int quant = UNROLL * JS;
         chunk = (n + target - 1) / target;
         chunk = ((chunk + quant - 1) / quant) * quant;
         G = (n + chunk - 1) / chunk;
-        P = G + 1;
+        P = G + g->corr_max_pieces;
         part_elems = (size_t)bmax * P;
         if (getenv("BL_PERSISTENT") && p->precision == BL_PREC_MIXED) pl = plan_iter(n, bmax);
         if (pl.NW > 0) part_elems = std::max(part_elems, 2 * (size_t)bmax * pl.NW);
@@ -592,18 +604,20 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
                     prof.end();
                 } else {
                     const int ipw = WG / JS;
-                    const dim3 grid(G + (bc + 3) / 4, (bc + ipw - 1) / ipw);
+                    const int pc0 = g->h_cpre[b0], npc = g->h_cpre[b0 + bc] - pc0;
+                    const dim3 grid(G + (npc + 3) / 4, (bc + ipw - 1) / ipw);
+                    const int *cpre = g->cpre.as<int>(), *cpv = g->cpv.as<int>();
                     hipEvent_t e0, e1;
                     prof.ext(0, &e0, &e1);
                     if (p->precision == BL_PREC_FP32)
                         hipExtLaunchKernelGGL((k_pairs32<4, false>), grid, dim3(WG), 0, s, e0, e1, 0, pos, n, b0, bc,
-                                              chunk, G, part, P, erp, eci, ew, p->weight, law);
+                                              chunk, G, part, P, erp, eci, ew, p->weight, law, cpre, cpv, pc0, npc);
                     else
                         hipExtLaunchKernelGGL((k_pairs32<4, true>), grid, dim3(WG), 0, s, e0, e1, 0, pos, n, b0, bc,
-                                              chunk, G, part, P, erp, eci, ew, p->weight, law);
+                                              chunk, G, part, P, erp, eci, ew, p->weight, law, cpre, cpv, pc0, npc);
                     prof.ext(1, &e0, &e1);
-                    hipExtLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, e0, e1, 0, part, P, pos, b0, bc, STEP,
-                                          eterm);
+                    hipExtLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, e0, e1, 0, part, P, G, cpre, pos, b0, bc,
+                                          STEP, eterm);
                 }
             }
         }

@@ -201,3 +201,31 @@ def test_rgg_250k_one_iteration_fp32_vs_fp64(bl):
         print(f"rgg250k {tag}: n={n} max|dx|/diam={err:.3e} energy rel={rel_e:.2e} "
               f"{st.seconds*1e3:.1f} ms (fp64 {s64.seconds*1e3:.1f} ms)")
         assert err <= TOL and rel_e <= 1e-5
+
+
+@pytest.mark.parametrize("law,prec", [(2, 0), (8, 0), (2, 2), (8, 2)])
+def test_hub_corrections_in_pieces(bl, oracle, law, prec):
+    """Vertices with thousands of neighbours: their corrections are split over several waves
+    (pieces of 256 entries) and summed per vertex; one iteration against the oracle."""
+    rng = np.random.default_rng(21)
+    n = 3000
+    e = [rng.integers(0, n, size=(4 * n, 2))]
+    for h in (0, 7, 1500, n - 1):  # hubs of 300..2000 neighbours, in different batches
+        others = rng.choice(n, int(rng.integers(300, 2000)), replace=False)
+        e.append(np.stack([np.full(len(others), h), others], 1))
+    e = np.concatenate(e)
+    e = e[e[:, 0] != e[:, 1]]
+    rows = np.concatenate([e[:, 0], e[:, 1]])
+    cols = np.concatenate([e[:, 1], e[:, 0]])
+    rc = np.unique(np.stack([rows, cols], 1), axis=0)
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(rc[:, 0], minlength=n), out=rp[1:])
+    rp, ci = rp.astype(np.int32), rc[:, 1].astype(np.int32)
+    assert np.diff(rp).max() > 1000
+    w = rng.uniform(0.5, 2.0, size=len(ci)).astype(np.float32)
+    init = rng.random((n, 2))
+    ref, _, _ = oracle.minibatch(rp, ci, init, 1, 256, law=law, weight=1.0, w_edge=w)
+    with bl.Graph(rp, ci, w) as g:
+        print(f"[case] hub corrections law {law} precision {prec}", flush=True)
+        xy, st = g.run(init, algo=law, iters=1, batch=256, precision=prec)
+    assert np.abs(xy - ref).max() <= TOL * diameter(ref)
