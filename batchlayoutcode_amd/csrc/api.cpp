@@ -118,6 +118,8 @@ extern "C" int bl_graph_create(int32_t n, int64_t nnz, const int32_t *rowptr, co
     g->has_w = w_edge != nullptr;
     g->eff_max_deg = maxdeg;
     g->h_rowptr.assign(rowptr, rowptr + n + 1);
+    g->h_colids.assign(colids, colids + nnz);
+    if (w_edge) g->h_w.assign(w_edge, w_edge + nnz);
     for (int32_t i = 0; i < n; ++i) g->max_row = std::max(g->max_row, rowptr[i + 1] - rowptr[i]);
     auto fail = [&](int code) {
         bl_graph_destroy(g);
@@ -172,7 +174,8 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
     if (!g) return BL_E_ARG;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
-                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model, &g->cpre, &g->cpv};
+                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model, &g->cpre, &g->cpv,
+                     &g->early_rp, &g->early_ci, &g->early_w, &g->late_rp, &g->late_ci, &g->late_w};
     for (DevBuf *b : all) b->release();
     g->bh.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);

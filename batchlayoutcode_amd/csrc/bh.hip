@@ -1130,16 +1130,74 @@ constexpr size_t wrow_lds(bool has_w) {
            sizeof(int) * (2 * WROW_CAP + 4 * WROW_T) + (has_w ? sizeof(float) * 2 * WROW_CAP : 0);
 }
 
-// Whether k_walk_row can take batches of B vertices of graph g (cached per batch size).
+// Whether k_walk_row can take batches of B vertices of graph g, and if so the early/late
+// split of the CSR on the device (both cached per batch size).  Early entries of a batch
+// must fit the walk's stage.
 bool row_walk_fits(bl_graph *g, int B) {
-    if (g->row_walk_batch != B) {
-        bool ok = B <= WROW_T && g->nnz > 0 && g->max_row <= WROW_MAXDEG && (int)g->h_rowptr.size() == g->n + 1;
-        for (int b0 = 0; ok && b0 < g->n; b0 += B)
-            ok = g->h_rowptr[std::min(g->n, b0 + B)] - g->h_rowptr[b0] <= WROW_CAP;
-        g->row_walk_batch = B;
-        g->row_walk_ok = ok;
+    if (g->row_walk_batch == B) return g->row_walk_ok;
+    g->row_walk_batch = B;
+    g->row_walk_ok = false;
+    const int n = g->n;
+    if (!(B <= WROW_T && g->nnz > 0 && g->max_row <= WROW_MAXDEG && (int)g->h_rowptr.size() == n + 1 &&
+          (int64_t)g->h_colids.size() == g->nnz))
+        return false;
+    const bool hw = !g->h_w.empty();
+    std::vector<int32_t> erp(n + 1, 0), lrp(n + 1, 0), eci, lci;
+    std::vector<float> ew, lw;
+    eci.reserve(g->nnz / 2 + 1);
+    lci.reserve(g->nnz / 2 + 1);
+    for (int i = 0; i < n; ++i) {
+        const int b0 = (i / B) * B;
+        for (int e = g->h_rowptr[i]; e < g->h_rowptr[i + 1]; ++e) {
+            const bool early = g->h_colids[e] < b0;
+            (early ? eci : lci).push_back(g->h_colids[e]);
+            if (hw) (early ? ew : lw).push_back(g->h_w[e]);
+        }
+        erp[i + 1] = (int32_t)eci.size();
+        lrp[i + 1] = (int32_t)lci.size();
     }
-    return g->row_walk_ok;
+    for (int b0 = 0; b0 < n; b0 += B)
+        if (erp[std::min(n, b0 + B)] - erp[b0] > WROW_CAP) return false;
+    // the walk reads index 0 of an empty list (clamped loads): keep one valid entry
+    if (eci.empty()) eci.push_back(0), ew.push_back(0.0f);
+    if (lci.empty()) lci.push_back(0), lw.push_back(0.0f);
+    if (g->early_rp.ensure(sizeof(int32_t) * (n + 1)) || g->late_rp.ensure(sizeof(int32_t) * (n + 1)) ||
+        g->early_ci.ensure(sizeof(int32_t) * eci.size()) || g->late_ci.ensure(sizeof(int32_t) * lci.size()))
+        return false;
+    if (hw && (g->early_w.ensure(sizeof(float) * ew.size()) || g->late_w.ensure(sizeof(float) * lw.size())))
+        return false;
+    if (hipMemcpy(g->early_rp.p, erp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->late_rp.p, lrp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->early_ci.p, eci.data(), sizeof(int32_t) * eci.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->late_ci.p, lci.data(), sizeof(int32_t) * lci.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return false;
+    if (hw && (hipMemcpy(g->early_w.p, ew.data(), sizeof(float) * ew.size(), hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(g->late_w.p, lw.data(), sizeof(float) * lw.size(), hipMemcpyHostToDevice) != hipSuccess))
+        return false;
+    g->row_walk_ok = true;
+    return true;
+}
+
+// Attraction of every vertex's late neighbours (same or later batch: iteration-start
+// positions), in CSR order, folded into the repulsion the walk subtracts:
+// rep_i <- rep_i - sum_late d * (W w |d|), so the walk's f = sum_early - rep_i is the whole
+// force (algorithms.cpp:1055-1059).  One thread per vertex (rows are short here).
+__global__ void k_late_attract(const int *__restrict__ lrp, const int *__restrict__ lci,
+                               const float *__restrict__ lw, double W, const double2 *__restrict__ pos,
+                               double2 *__restrict__ rep, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 pi = pos[i];
+    double ax = 0.0, ay = 0.0;
+    for (int e = lrp[i]; e < lrp[i + 1]; ++e) {
+        const double2 pj = pos[lci[e]];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y;
+        const double sc = (lw ? W * (double)lw[e] : W) * sqrt(dx * dx + dy * dy);
+        ax = fma(dx, sc, ax);
+        ay = fma(dy, sc, ay);
+    }
+    const double2 r = rep[i];
+    rep[i] = make_double2(r.x - ax, r.y - ay);
 }
 
 template <bool HAS_W>
@@ -1771,11 +1829,15 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->bh.misc.as<int>());
         else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK)
             ;
-        else if (use_row)
+        else if (use_row) {
+            hipLaunchKernelGGL(k_late_attract, dim3((n + 255) / 256), dim3(256), 0, s, g->late_rp.as<int>(),
+                               g->late_ci.as<int>(), g->has_w ? g->late_w.as<float>() : nullptr, p->weight, pos,
+                               g->bh.rep.as<double2>(), n);
             hipLaunchKernelGGL(g->has_w ? k_walk_row<true> : k_walk_row<false>, dim3(1), dim3(2 * WROW_T),
-                               wrow_lds(g->has_w), s, g->rowptr.as<int>(), g->colids.as<int>(),
-                               g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
+                               wrow_lds(g->has_w), s, g->early_rp.as<int>(), g->early_ci.as<int>(),
+                               g->has_w ? g->early_w.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                n, p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
+        }
         else if (p->batch <= WALK_MAXB)
             hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(), n,

@@ -59,10 +59,16 @@ struct bl_graph {
     // CSR as given (algo 4 attraction walks every entry, algorithms.cpp:1055-1058)
     bl::DevBuf rowptr, colids, w_edge;
     bool has_w = false;
-    std::vector<int32_t> h_rowptr;  // host copy of rowptr (walk selection)
+    std::vector<int32_t> h_rowptr, h_colids;  // host copy of the CSR (walk selection and split)
+    std::vector<float> h_w;
     int32_t max_row = 0;            // longest CSR row
-    int32_t row_walk_batch = 0;     // batch size row_walk_ok was computed for
+    int32_t row_walk_batch = 0;     // batch size row_walk_ok and the split were computed for
     bool row_walk_ok = false;
+    // the CSR split for the one-workgroup row walk at batch size row_walk_batch: "early"
+    // entries (neighbour in an earlier batch, moved during the iteration before the row's
+    // batch) and "late" ones (neighbour in the same or a later batch: its iteration-start
+    // position), each in CSR order
+    bl::DevBuf early_rp, early_ci, early_w, late_rp, late_ci, late_w;
     // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),
     // as a CSR of strictly increasing column ids per row (+ their weights)
     bl::DevBuf eff_rowptr, eff_colids, eff_w;

@@ -108,17 +108,32 @@ def test_algo4_fp32_first_and_teacher_forced(bl, fx):
 
 
 def test_algo4_final_energy_600_iterations(bl):
+    """600 iterations of algo 4 on the 64x64 grid from the fixture init and 17 1-ulp
+    perturbations of it (the band fixture's scheme).  The trajectory is chaotic: two walks that
+    differ only in the order of partial sums agree to 4e-16 after one iteration, 3e-12 after
+    20 and not at all after 40, so each final energy is a sample.  Measured over these 18
+    starts, relative to the mean of the reference's six runs: the reference-order FP64 path
+    -0.7% +- 1.6% (runs -15%..+14%), the default path -5.9% +- 1.7% (runs -24%..+3%).  Bars:
+    the mean within 10% of the reference mean, every run within 30%."""
     band = golden("band_a4_grid64_b256_600.npz")
-    e_ref = float(band["E"][0])
-    tol = max(0.01, (band["E"].max() - band["E"].min()) / e_ref)
+    e_ref = float(band["E"].mean())
     init = golden("traj_a4_grid64_b256.npz")["init"]
     rp, ci = bl.mtx_read(golden_input("grid64x64.mtx"))
+    es = []
     with bl.Graph(rp, ci) as g:
-        print("[case] grid64 algo 4 600 iterations", flush=True)
-        xy, st = g.run(init, algo=4, iters=600, batch=256)
-    rel = abs(st.energy - e_ref) / e_ref
-    print(f"algo 4: E={st.energy:.6e} ref={e_ref:.6e} rel={rel:.3e} tol={tol:.3e} ({st.seconds*1e3:.1f} ms)")
-    assert rel <= tol
+        print("[case] grid64 algo 4 600 iterations, 18 starts", flush=True)
+        for p in range(18):
+            xy0 = init.copy()
+            if p > 0:
+                rng = np.random.default_rng(100 + p)
+                xy0 = np.nextafter(xy0, xy0 + rng.choice([-1.0, 1.0], size=xy0.shape))
+            xy, st = g.run(xy0, algo=4, iters=600, batch=256)
+            es.append(st.energy / e_ref - 1)
+    es = np.array(es)
+    print(f"algo 4: mean {es.mean():+.4f} sd {es.std(ddof=1):.4f} range [{es.min():+.3f}, {es.max():+.3f}] "
+          f"({st.seconds*1e3:.1f} ms/run)")
+    assert abs(es.mean()) <= 0.10
+    assert np.abs(es).max() <= 0.30
 
 
 def _hubby_graph(n, hubs, seed):
@@ -156,16 +171,22 @@ def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo):
     assert abs(out["grid_E"] - out["single_E"]) <= 1e-10 * out["single_E"]
 
 
-@pytest.mark.parametrize("weighted,batch,algo,prec", [(False, 256, 4, 0), (True, 256, 4, 0), (False, 100, 4, 0),
-                                                      (True, 37, 4, 0), (False, 256, 40, 0), (False, 256, 4, 2)])
-def test_row_walk_matches_entry_split_walk(bl, monkeypatch, weighted, batch, algo, prec):
-    # short rows and batches of at most 256 vertices take the one-thread-per-vertex walk; it
-    # must agree with the entry-split walk up to the order of partial sums of split rows
-    rp, ci = graphs.csr_from_edges(*graphs.rgg(20000, seed=5))
+@pytest.mark.parametrize("kind,weighted,batch,algo,prec", [("rgg", False, 256, 4, 0), ("rgg", True, 256, 4, 0),
+                                                           ("rgg", False, 100, 4, 0), ("rgg", True, 37, 4, 0),
+                                                           ("rgg", False, 256, 40, 0), ("rgg", False, 256, 4, 2),
+                                                           ("grid", False, 256, 4, 0)])
+def test_row_walk_matches_entry_split_walk(bl, monkeypatch, kind, weighted, batch, algo, prec):
+    # short rows and batches of at most 256 vertices take the one-thread-per-vertex walk (the
+    # attraction of neighbours in the same or later batches summed beforehand); it must agree
+    # with the entry-split walk up to the order of partial sums
+    if kind == "grid":
+        rp, ci = graphs.csr_from_edges(*graphs.grid(64, 64))
+    else:
+        rp, ci = graphs.csr_from_edges(*graphs.rgg(20000, seed=5))
     n = len(rp) - 1
     w = np.random.default_rng(7).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
     x0 = np.random.default_rng(8).random((n, 2))
-    print(f"[case] row walk weighted {weighted} batch {batch} algo {algo} precision {prec}", flush=True)
+    print(f"[case] row walk {kind} weighted {weighted} batch {batch} algo {algo} precision {prec}", flush=True)
     out = {}
     for mode in ("single", "pf"):
         monkeypatch.setenv("BL_WALK", mode)
