@@ -1389,7 +1389,7 @@ __global__ __launch_bounds__(2 * WROW_T) void k_walk_row(const int *__restrict__
 //      were written before a barrier that every reader has passed since.
 // The results differ from the one-workgroup walk only in the order of the partial sums.
 constexpr int GW_T = 1024;
-constexpr int GW_G = 32;
+constexpr int GW_G = 16;  // 8..32 measured on the R-MAT walk: 16 is fastest
 constexpr int GW_MIN_ENTRIES = 4096;  // mean CSR entries per batch from which it is used
 
 struct GridWalk {
