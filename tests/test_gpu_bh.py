@@ -174,13 +174,23 @@ def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo):
 @pytest.mark.parametrize("kind,weighted,batch,algo,prec", [("rgg", False, 256, 4, 0), ("rgg", True, 256, 4, 0),
                                                            ("rgg", False, 100, 4, 0), ("rgg", True, 37, 4, 0),
                                                            ("rgg", False, 256, 40, 0), ("rgg", False, 256, 4, 2),
-                                                           ("grid", False, 256, 4, 0)])
+                                                           ("grid", False, 256, 4, 0), ("small", True, 256, 4, 0),
+                                                           ("small", False, 1, 4, 0), ("isolated", False, 64, 4, 0)])
 def test_row_walk_matches_entry_split_walk(bl, monkeypatch, kind, weighted, batch, algo, prec):
     # short rows and batches of at most 256 vertices take the one-thread-per-vertex walk (the
     # attraction of neighbours in the same or later batches summed beforehand); it must agree
     # with the entry-split walk up to the order of partial sums
     if kind == "grid":
         rp, ci = graphs.csr_from_edges(*graphs.grid(64, 64))
+    elif kind == "small":  # one partial batch at B=256, or a batch per vertex
+        rp, ci = graphs.csr_from_edges(*graphs.rgg(120, seed=6))
+    elif kind == "isolated":  # a random geometric graph with a sparse radius: many empty rows
+        n0 = 3000
+        pts = np.random.default_rng(9).random((n0, 2))
+        d2 = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+        a, b = np.nonzero(np.triu(d2 < 0.012 ** 2, 1))
+        rp, ci = graphs.csr_from_edges(n0, np.stack([a, b], 1))
+        assert (np.diff(rp) == 0).sum() > 100
     else:
         rp, ci = graphs.csr_from_edges(*graphs.rgg(20000, seed=5))
     n = len(rp) - 1
