@@ -44,6 +44,27 @@ struct BatchedArgs {
     int *queue;
 };
 
+typedef __attribute__((address_space(3))) const double lds_double;
+
+// Dense repulsion of position pi against positions [j0, j1) of `src` (x, y interleaved, in LDS
+// or global memory; with an LDS-typed pointer the loads are LDS broadcasts instead of generic
+// loads).
+template <bool REFINE, class Src>
+__device__ __forceinline__ void dense_range(Src src, int j0, int j1, double2 pi, double &ax, double &ay) {
+    int u = j0;
+    for (; u + 8 <= j1; u += 8) {  // eight positions in flight per load batch
+        double qx[8], qy[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            qx[k] = src[2 * (u + k)];
+            qy[k] = src[2 * (u + k) + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) pair_rep<REFINE>(qx[k], qy[k], pi.x, pi.y, ax, ay);
+    }
+    for (; u < j1; ++u) pair_rep<REFINE>(src[2 * u], src[2 * u + 1], pi.x, pi.y, ax, ay);
+}
+
 template <bool REFINE>
 __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *red, double2 *fb, double *eb) {
     const int tid = threadIdx.x;
@@ -74,18 +95,10 @@ __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *r
                 const bool act = slot < bc;
                 const double2 pi = gp[b0 + (act ? slot : 0)];
                 double ax = 0.0, ay = 0.0;
-                int u = j0;
-                for (; u + 8 <= j1; u += 8) {  // eight positions in flight per load batch
-                    double2 q[8];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) q[k] = gp[u + k];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) pair_rep<REFINE>(q[k].x, q[k].y, pi.x, pi.y, ax, ay);
-                }
-                for (; u < j1; ++u) {
-                    const double2 q = gp[u];
-                    pair_rep<REFINE>(q.x, q.y, pi.x, pi.y, ax, ay);
-                }
+                if (in_lds)
+                    dense_range<REFINE>((lds_double *)P, j0, j1, pi, ax, ay);
+                else
+                    dense_range<REFINE>(reinterpret_cast<const double *>(gp), j0, j1, pi, ax, ay);
                 red[tid] = make_double2(ax, ay);
                 __syncthreads();
                 if (js == 0 && act) {
