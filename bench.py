@@ -248,7 +248,11 @@ def main():
                     unit="TFLOP/s", frac=achieved / PEAK_FP64_TFLOPS, traffic=measured_traffic(),
                     mean_launch_us=pair_ms * 1e3,
                     launches=sp_prof.kernel_launches[0], flops_per_launch=flops_per_launch,
-                    flops_per_pair=FLOPS_PER_PAIR)
+                    flops_per_pair=FLOPS_PER_PAIR,
+                    note="bound by the FP64 vector (VALU) pipes: 2-D pair terms have no dense contraction for "
+                         "MFMA; the peak is MI355X's FP64 vector rate, the same 78.6 TFLOP/s as its published "
+                         "dense FP64 MFMA figure; traffic is HBM bytes per launch from FETCH_SIZE and WRITE_SIZE "
+                         "scaled by a known-size copy")
     achieved32 = flops_per_launch / (pair32_ms / 1e3) / 1e12
     out = dict(metric=METRIC, value=n * K / secs, unit="vertex-updates/s", n_gpus=1, steps=K, warmup=W,
                ms_per_step=secs * 1e3 / K, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
