@@ -181,14 +181,15 @@ constexpr size_t QOFF = (sizeof(MlpLds) + 15) & ~(size_t)15;
 struct FarQueue {
     float feat[QCAP][MLP_IN];
     float mono[QCAP][2];  // the cell's monopole m (c - v)/|c - v|^2
-    int owner[QCAP];      // lane of the query
     float corr[64][2];    // model correction of the evaluated tile
 };
 constexpr size_t LEARNED_LDS = QOFF + sizeof(FarQueue) * (WG / 64);
 
 // Evaluates the first 64 queued cells and adds (f7 y0) mono + (f7 y1) mono_perp to their
-// owners, each lane summing its own cells in queue order.
-__device__ void far_flush(const MlpLds &L, FarQueue &Q, int count, int lane, double &fx, double &fy) {
+// owners, each lane summing its own cells in queue order: `mine` has bit s set when the
+// lane queued slot s (a lane visits only its own slots, not all 64).
+__device__ void far_flush(const MlpLds &L, FarQueue &Q, int count, int lane, uint64_t mine, double &fx,
+                          double &fy) {
     auto feat = [&](int s, int f) { return s < count ? Q.feat[s][f] : 0.0f; };
     f32x4 Y[4];
     mlp_forward(L, feat, Y, lane);
@@ -203,12 +204,10 @@ __device__ void far_flush(const MlpLds &L, FarQueue &Q, int count, int lane, dou
         }
     }
     __builtin_amdgcn_wave_barrier();
-    const int lim = min(count, 64);
-    for (int s = 0; s < lim; ++s) {
-        if (Q.owner[s] == lane) {
-            fx += (double)Q.corr[s][0];
-            fy += (double)Q.corr[s][1];
-        }
+    for (uint64_t m = mine; m; m &= m - 1) {
+        const int s = __builtin_ctzll(m);
+        fx += (double)Q.corr[s][0];
+        fy += (double)Q.corr[s][1];
     }
     __builtin_amdgcn_wave_barrier();
 }
@@ -253,6 +252,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
     }
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     int qn = 0;  // queued cells of this wave (learned mode)
+    uint64_t mlo = 0, mhi = 0;  // this lane's queue slots 0..63 and 64..127 (learned mode)
     if (lane == 0) {
         stk_node[w][0] = root;
         stk_mask[w][0] = m0;
@@ -294,13 +294,13 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
                     const double s = (double)cl.m * inv_r2<true>(dx, dy);
                     Q.mono[slot][0] = (float)(dx * s);
                     Q.mono[slot][1] = (float)(dy * s);
-                    Q.owner[slot] = lane;
+                    if (slot < 64) mlo |= 1ull << slot; else mhi |= 1ull << (slot - 64);
                 }
                 qn += __popcll(accm);
                 __builtin_amdgcn_wave_barrier();
                 if (qn >= 64) {
                     FarQueue &Q = Qs[w];
-                    far_flush(*L, Q, 64, lane, fx, fy);
+                    far_flush(*L, Q, 64, lane, mlo, fx, fy);
                     // move the remaining qn - 64 (< 64) entries to the front
                     const int rest = qn - 64;
                     if (lane < rest) {
@@ -308,9 +308,10 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
                         for (int f = 0; f < MLP_IN; ++f) Q.feat[lane][f] = Q.feat[64 + lane][f];
                         Q.mono[lane][0] = Q.mono[64 + lane][0];
                         Q.mono[lane][1] = Q.mono[64 + lane][1];
-                        Q.owner[lane] = Q.owner[64 + lane];
                     }
                     qn = rest;
+                    mlo = mhi;
+                    mhi = 0;
                     __builtin_amdgcn_wave_barrier();
                 }
             }
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         }
     }
     if constexpr (MODE == TRAV_LEARNED) {
-        if (qn > 0) far_flush(*L, Qs[w], qn, lane, fx, fy);
+        if (qn > 0) far_flush(*L, Qs[w], qn, lane, mlo, fx, fy);
     }
     if (valid) out[qid[k]] = make_double2(fx, fy);
 }

@@ -51,16 +51,23 @@ __device__ __forceinline__ bf16x4 frag_a(const short *M, int ld, int r0, int k0,
 
 // GELU, tanh form: g(z) = z/2 (1 + tanh(c (z + 0.044715 z^3))), c = sqrt(2/pi).
 // tanh(u) = 1 - 2/(exp(2u) + 1) with the hardware exp2 and reciprocal.
-__device__ __forceinline__ float gelu_t(float z) {
-    const float u = 0.7978845608028654f * fmaf(0.044715f * z, z * z, z);
-    const float e = __builtin_amdgcn_exp2f(2.8853900817779268f * u);  // exp(2u)
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// GELU in tanh form, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z + 0.044715 z^3), evaluated as
+// z - z / (1 + e) with e = exp(2u) = 2^(z (GA + GB z^2)): 0.5 (1 + tanh u) = 1 - 1/(1 + e).
+// Six VALU operations and two transcendentals; z -> -inf gives e -> 0 and exactly 0, z -> inf
+// gives e -> inf and z.
+constexpr float GA = 2.3022082f;   // 2 sqrt(2/pi) / ln 2
+constexpr float GB = 0.10294324f;  // GA * 0.044715
+__device__ __forceinline__ float gelu_r(float z) {  // 1 / (1 + exp(2u))
+    const float e = __builtin_amdgcn_exp2f(z * fmaf(GB, z * z, GA));
+    return __builtin_amdgcn_rcpf(e + 1.0f);
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + gelu_t(z)); }
+__device__ __forceinline__ float gelu(float z) { return fmaf(-z, gelu_r(z), z); }
+// d/dz: s + z * 2 s r u'(z) with s = 1 - r = 0.5 (1 + tanh u), 2 s r = 0.5 (1 - tanh^2 u) and
+// u'(z) = sqrt(2/pi) (1 + 3 * 0.044715 z^2)
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float t = gelu_t(z);
-    const float du = 0.7978845608028654f * fmaf(3.0f * 0.044715f * z, z, 1.0f);
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+    const float r = gelu_r(z), sg = 1.0f - r;
+    const float du = fmaf(0.10703222f, z * z, 0.7978845608f);  // sqrt(2/pi) (1 + 0.134145 z^2)
+    return fmaf(z * (2.0f * sg * r), du, sg);
 }
 
 // Weights of the model in LDS as BF16, row-major, padded to MFMA tiles.
