@@ -174,6 +174,31 @@ __device__ __forceinline__ void far_features(const FarCell &cl, double dx, doubl
     f[7] = (float)(t * t);
 }
 
+// The same features in FP32 with hardware reciprocal, reciprocal square root and log2 (about
+// 1 ulp each): the learned traversal's model takes them as BF16 inputs, so FP32 is ample,
+// and FP64 divisions, square root and log2 cost every wave in which any lane accepts a cell.
+// Training samples (k_far_samples) and the quadrupole term use far_features.
+__device__ __forceinline__ void far_features32(const FarCell &cl, double dx, double dy, double r2, double D,
+                                               float *f) {
+    const float irho = __builtin_amdgcn_rsqf((float)r2);
+    const float ux = (float)dx * irho, uy = (float)dy * irho, wx = -uy, wy = ux;
+    const float m = (float)cl.m;
+    const float s = __builtin_amdgcn_rcpf(m * (float)r2);
+    const float mxx = (float)cl.mxx, mxy = (float)cl.mxy, myy = (float)cl.myy;
+    const float suu = (ux * ux * mxx + 2.0f * ux * uy * mxy + uy * uy * myy) * s;
+    const float suw = (ux * wx * mxx + (ux * wy + uy * wx) * mxy + uy * wy * myy) * s;
+    const float sww = (wx * wx * mxx + 2.0f * wx * wy * mxy + wy * wy * myy) * s;
+    const float t = (float)D * irho;
+    f[0] = t;
+    f[1] = suu;
+    f[2] = suw;
+    f[3] = sww;
+    f[4] = suu - sww;
+    f[5] = __builtin_amdgcn_logf(m) * (1.0f / 16.0f);
+    f[6] = __builtin_amdgcn_rcpf(m);
+    f[7] = t * t;
+}
+
 enum TravMode : int { TRAV_MONO = 0, TRAV_QUAD = 1, TRAV_LEARNED = 2 };
 constexpr size_t QOFF = (sizeof(MlpLds) + 15) & ~(size_t)15;
 
@@ -290,7 +315,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
                 if (act && acc) {
                     const int slot = qn + __popcll(accm & lt);
                     FarQueue &Q = Qs[w];
-                    far_features(cl, dx, dy, r2, D, Q.feat[slot]);
+                    far_features32(cl, dx, dy, r2, D, Q.feat[slot]);
                     const double s = (double)cl.m * inv_r2<true>(dx, dy);
                     Q.mono[slot][0] = (float)(dx * s);
                     Q.mono[slot][1] = (float)(dy * s);
