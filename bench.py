@@ -49,8 +49,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n, rp, ci, init, batch, threads):
-    """One iteration of the workload on the host cores (bounded sample)."""
+def cpu_baseline(n, rp, ci, init, batch, threads, algo=2, iters=1):
+    """`iters` iterations of `algo` on the workload's graph on the host cores (bounded sample):
+    the reference built from source when it is there, else the C oracle (algo 2 only)."""
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "ref_fast")
     if os.path.exists(ref_bin):
         from batchlayoutcode_amd import graphs
@@ -63,7 +64,7 @@ def cpu_baseline(n, rp, ci, init, batch, threads):
             graphs.write_mtx(mtx, n, e)
             graphs.write_initf(initf, init)
             t0 = time.time()
-            out = subprocess.run([ref_bin, "run", mtx, initf, "2", "1", str(batch), "1.2", "1.0", "0.01",
+            out = subprocess.run([ref_bin, "run", mtx, initf, str(algo), str(iters), str(batch), "1.2", "1.0", "0.01",
                                   str(threads)], capture_output=True, text=True, cwd=td, check=True).stdout
             wall = time.time() - t0
         secs = None
@@ -72,17 +73,20 @@ def cpu_baseline(n, rp, ci, init, batch, threads):
                 secs = float(ln.split(":")[-1])
         if secs is None:
             secs = wall
-        return dict(value=n / secs, unit="vertex-updates/s", cores=threads, kind="reference",
-                    sample=f"1 iteration of algo 2 on the same {n}-vertex graph, -batch {batch}, "
+        what = "algo 2" if algo == 2 else f"algo {algo} (-bht 1.2)"
+        return dict(value=n * iters / secs, unit="vertex-updates/s", cores=threads, kind="reference",
+                    sample=f"{iters} iteration(s) of {what} on the same {n}-vertex graph, -batch {batch}, "
                            f"-threads {threads} (reference built from source with its Makefile flags); "
                            f"{secs:.2f} s")
+    if algo != 2:
+        return None
     from oracle import pyoracle
 
     t0 = time.time()
-    pyoracle.minibatch(rp, ci, init, 1, batch, threads=threads)
+    pyoracle.minibatch(rp, ci, init, iters, batch, threads=threads)
     secs = time.time() - t0
-    return dict(value=n / secs, unit="vertex-updates/s", cores=threads, kind="port",
-                sample=f"1 iteration of algo 2 on the same {n}-vertex graph with the C oracle, "
+    return dict(value=n * iters / secs, unit="vertex-updates/s", cores=threads, kind="port",
+                sample=f"{iters} iteration(s) of algo 2 on the same {n}-vertex graph with the C oracle, "
                        f"{threads} threads; {secs:.2f} s")
 
 
@@ -271,6 +275,12 @@ def main():
         log(f"cpu baseline on {threads} threads ...")
         out["cpu_baseline"] = cpu_baseline(n, rp, ci, init, B, threads)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.3e} vertex-updates/s")
+        # the reference's Barnes-Hut (algo 4) on the same graph; the GPU's algo 4 on it is
+        # far_field.modes.reference_bh_unit_vector
+        bh = cpu_baseline(n, rp, ci, init, B, threads, algo=4, iters=10)
+        if bh is not None:
+            out["cpu_baseline_algo4"] = bh
+            log(f"cpu baseline algo 4: {bh['value']:.3e} vertex-updates/s")
     print(json.dumps(out), flush=True)
 
 
