@@ -1387,9 +1387,14 @@ __global__ __launch_bounds__(2 * WROW_T) void k_walk_row(const int *__restrict__
 //      all workgroups); workgroup 0 also writes them to memory.  The next batch reads the
 //      positions of this batch from that LDS copy, and older positions from memory, which
 //      were written before a barrier that every reader has passed since.
+// Loads run a batch ahead: the next batch's rowptr slice, own positions and each thread's
+// first GW_K column ids are loaded before the barrier, the neighbour positions they address
+// right after it (older batches are then final in memory; this batch's moves are patched
+// from the LDS copy), so that the next batch starts on data already in registers.
 // The results differ from the one-workgroup walk only in the order of the partial sums.
 constexpr int GW_T = 1024;
-constexpr int GW_G = 16;  // 8..32 measured on the R-MAT walk: 16 is fastest
+constexpr int GW_G = 24;  // 8..32 measured on the R-MAT walk: 24 and 32 are fastest
+constexpr int GW_K = 4;  // entries per thread loaded a batch ahead
 constexpr int GW_MIN_ENTRIES = 4096;  // mean CSR entries per batch from which it is used
 
 struct GridWalk {
@@ -1414,6 +1419,8 @@ __device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_doub
 
 __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
     constexpr int T = GW_T;
+    constexpr int PR = (WALK_MAXB + T) / T;      // rowptr values per thread (a batch has bc + 1)
+    constexpr int PP = (WALK_MAXB + T - 1) / T;  // vertices per thread
     __shared__ double2 head[T], tail[T], S[T];
     __shared__ int hrow[T], trow[T];
     __shared__ int rps[WALK_MAXB + 1];
@@ -1429,11 +1436,56 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
     const int nb = (n + B - 1) / B;
     if (t == 0) bad = 0;
     int pb0 = 0, pb1 = 0;  // the previous batch, whose moved positions are in newpos
+    // Loaded a batch ahead, so that no load sits on a batch's critical path: the next batch's
+    // rowptr slice and own positions (nrp, nbp), and the column ids, weights and neighbour
+    // positions of this thread's first GW_K entries of it (cid, wf, pj).
+    int nrp[PR];
+    double2 nbp[PP];
+    int cid[GW_K];
+    float wf[GW_K];
+    double2 pj[GW_K];
+    // prologue: batch 0
+    {
+        const int cc = min(n, B);
+#pragma unroll
+        for (int k = 0; k < PR; ++k) nrp[k] = t + k * T <= cc ? a.rowptr[t + k * T] : 0;
+#pragma unroll
+        for (int k = 0; k < PP; ++k) nbp[k] = t + k * T < cc ? a.pos[t + k * T] : make_double2(0.0, 0.0);
+        const int E0 = a.rowptr[0], E1 = a.rowptr[cc];
+        const int Cw = (E1 - E0 + G - 1) / G;
+        const int Sw = min(E1, E0 + w * Cw), Ew = min(E1, Sw + Cw);
+        const int c = max(1, (Cw + T - 1) / T);
+        const int s = min(Ew, Sw + t * c), e = min(Ew, s + c);
+#pragma unroll
+        for (int g = 0; g < GW_K; ++g) {
+            cid[g] = s + g < e ? a.colids[s + g] : -1;
+            wf[g] = (a.w_edge && s + g < e) ? a.w_edge[s + g] : 1.0f;
+        }
+#pragma unroll
+        for (int g = 0; g < GW_K; ++g) pj[g] = cid[g] < 0 ? make_double2(0.0, 0.0) : a.pos[cid[g]];
+    }
     for (int b = 0; b < nb; ++b) {
         const int b0 = b * B, b1 = min(n, b0 + B), bc = b1 - b0, par = b & 1;
-        for (int k = t; k <= bc; k += T) rps[k] = a.rowptr[b0 + k];
-        for (int k = t; k < bc; k += T) bpos[k] = a.pos[b0 + k];
+        const bool more = b + 1 < nb;
+#pragma unroll
+        for (int k = 0; k < PR; ++k)
+            if (t + k * T <= bc) rps[t + k * T] = nrp[k];
+#pragma unroll
+        for (int k = 0; k < PP; ++k)
+            if (t + k * T < bc) bpos[t + k * T] = nbp[k];
         __syncthreads();
+        // the next batch's entry range, for loading its column ids before the barrier; as
+        // vector loads of values equal in all lanes (a uniform load would become a scalar
+        // load, whose wait counter the LDS accesses below share)
+        const int ncc = more ? min(n, b1 + B) - b1 : 0;
+        int nE0, nE1;
+        {
+            int va, ve;
+            __asm__("v_mov_b32 %0, %1" : "=v"(va) : "s"(b1));
+            __asm__("v_mov_b32 %0, %1" : "=v"(ve) : "s"(b1 + ncc));
+            nE0 = a.rowptr[va];
+            nE1 = a.rowptr[ve];
+        }
         auto row_of = [&](int entry) {  // last r in [b0, b1) with rowptr[r] <= entry
             int lo = b0, hi = b1;
             while (hi - lo > 1) {
@@ -1456,20 +1508,22 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
             int rend = rps[r + 1 - b0];
             double2 pi = bpos[r - b0];
             double ax = 0.0, ay = 0.0;
-            for (int k0 = s; k0 < e; k0 += WALK_G) {
-                int cid[WALK_G];
-                double2 pj[WALK_G];
-                double wk[WALK_G];
+            for (int k0 = s; k0 < e; k0 += GW_K) {
+                if (k0 != s) {  // entries beyond the first GW_K: loaded here
 #pragma unroll
-                for (int g = 0; g < WALK_G; ++g) cid[g] = k0 + g < e ? a.colids[k0 + g] : -1;
+                    for (int g = 0; g < GW_K; ++g) {
+                        cid[g] = k0 + g < e ? a.colids[k0 + g] : -1;
+                        wf[g] = (a.w_edge && k0 + g < e) ? a.w_edge[k0 + g] : 1.0f;
+                    }
 #pragma unroll
-                for (int g = 0; g < WALK_G; ++g) {
-                    const int j = cid[g];
-                    pj[g] = j < 0 ? make_double2(0.0, 0.0) : (j >= pb0 && j < pb1) ? newpos[j - pb0] : a.pos[j];
-                    wk[g] = (a.w_edge && k0 + g < e) ? a.W * (double)a.w_edge[k0 + g] : a.W;
+                    for (int g = 0; g < GW_K; ++g) pj[g] = cid[g] < 0 ? make_double2(0.0, 0.0) : a.pos[cid[g]];
                 }
+                // a neighbour in the previous batch: its move, which memory may not show yet
 #pragma unroll
-                for (int g = 0; g < WALK_G; ++g) {
+                for (int g = 0; g < GW_K; ++g)
+                    if (cid[g] >= pb0 && cid[g] < pb1) pj[g] = newpos[cid[g] - pb0];
+#pragma unroll
+                for (int g = 0; g < GW_K; ++g) {
                     const int k = k0 + g;
                     if (k >= e) break;
                     while (k >= rend) {  // row r complete inside this thread's run
@@ -1485,7 +1539,8 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
                         ax = ay = 0.0;
                     }
                     const double dx = pj[g].x - pi.x, dy = pj[g].y - pi.y;
-                    const double sc = wk[g] * sqrt(dx * dx + dy * dy);
+                    const double wk = a.w_edge ? a.W * (double)wf[g] : a.W;
+                    const double sc = wk * sqrt(dx * dx + dy * dy);
                     ax = fma(dx, sc, ax);
                     ay = fma(dy, sc, ay);
                 }
@@ -1552,14 +1607,40 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
                 }
             }
         }
-        // grid barrier: every workgroup's share of the batch is summed
-        __threadfence();
+        // the next batch's first entries of this thread: column ids and weights now, the
+        // positions they address after the barrier.  These loads, and those after the
+        // barrier, are unconditional with clamped indices (a slot past the thread's entries,
+        // or past the last batch, loads some valid entry and is never used): loads under a
+        // branch would make the compiler wait for every load in flight where it rejoins.
+        {
+            const int nCw = (nE1 - nE0 + G - 1) / G;
+            const int nSw = min(nE1, nE0 + w * nCw), nEw = min(nE1, nSw + nCw);
+            const int nc = max(1, (nCw + T - 1) / T);
+            const int ns = min(nEw, nSw + t * nc), ne = min(nEw, ns + nc);
+#pragma unroll
+            for (int g = 0; g < GW_K; ++g) {
+                const int kk = max(0, min(ns + g, ne - 1));
+                const int cv = a.colids[kk];
+                cid[g] = ns + g < ne ? cv : -1;
+            }
+            if (a.w_edge) {
+#pragma unroll
+                for (int g = 0; g < GW_K; ++g) wf[g] = a.w_edge[max(0, min(ns + g, ne - 1))];
+            }
+        }
+        // grid barrier: every workgroup's share of the batch is summed.  Each wave waits for
+        // its stores to reach the L2; thread 0's release below writes the L2 back, so one
+        // agent-scope fence per workgroup publishes the stores of all its waves (a fence in
+        // every wave would write the L2 back sixteen times).
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         if (t == 0) {
             __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned target = (unsigned)G * (unsigned)(b + 1);
             const long long t0 = wall_clock64();
-            while (__hip_atomic_load(a.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            // polled relaxed (an acquiring load would invalidate the L2 at every poll), then
+            // one acquire fence
+            while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                 if (__hip_atomic_load(a.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                     bad = 1;
                     break;
@@ -1571,26 +1652,52 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         // the same value in every thread: a uniform exit, not a per-lane branch out of the loop
         if (__builtin_amdgcn_readfirstlane(bad)) return;
-        // B. combine in workgroup order, subtract the repulsion, move (every workgroup)
-        for (int i = b0 + t; i < b1; i += T) {
+        // B. combine in workgroup order, subtract the repulsion, move (every workgroup).  The
+        // partials and repulsions are loaded first: loads complete in issue order, so the
+        // next batch's loads, issued after them, do not delay them.
+        double2 fv[PP], rv[PP];
+#pragma unroll
+        for (int k = 0; k < PP; ++k) {
+            const int ic = min(b0 + t + k * T, b1 - 1);
+            fv[k] = a.fbuf[ic];  // the row's sum when it lies in one workgroup's share
+            rv[k] = a.rep[ic];
+        }
+        // rows over several workgroups' shares (hubs): combined before the next batch's loads
+        // are issued, so that their loads are older than those too
+#pragma unroll
+        for (int k = 0; k < PP; ++k) {
+            const int i = b0 + t + k * T;
+            if (i >= b1) break;
             const int r0 = rps[i - b0], r1 = rps[i + 1 - b0];
-            double2 f = make_double2(0.0, 0.0);
-            if (r1 > r0) {
-                const int wa = (r0 - E0) / Cw, wb = (r1 - 1 - E0) / Cw;
-                if (wa == wb) {
-                    f = a.fbuf[i];
-                } else {
-                    f = a.wfirst[par * GW_G + wa] == i ? a.whead[par * GW_G + wa] : a.wtail[par * GW_G + wa];
-                    for (int q = wa + 1; q <= wb; ++q) f = add2(f, a.whead[par * GW_G + q]);
-                }
+            if (r1 <= r0) continue;
+            const int wa = (r0 - E0) / Cw, wb = (r1 - 1 - E0) / Cw;
+            if (wa != wb) {
+                double2 f = a.wfirst[par * GW_G + wa] == i ? a.whead[par * GW_G + wa] : a.wtail[par * GW_G + wa];
+                for (int q = wa + 1; q <= wb; ++q) f = add2(f, a.whead[par * GW_G + q]);
+                fv[k] = f;
             }
-            const double2 rr = a.rep[i];
-            f.x -= rr.x;  // f - calcRepForce(x_i), :1059
-            f.y -= rr.y;
+        }
+        // The next batch's rows and neighbour positions, in flight during the moves.  Every
+        // batch before this one is final in memory (written before this barrier); this
+        // batch's moves, made below, are taken from newpos when the next batch is summed.
+#pragma unroll
+        for (int k = 0; k < PR; ++k) nrp[k] = a.rowptr[min(b1 + t + k * T, n)];
+#pragma unroll
+        for (int k = 0; k < PP; ++k) nbp[k] = a.pos[min(b1 + t + k * T, n - 1)];
+#pragma unroll
+        for (int g = 0; g < GW_K; ++g) pj[g] = a.pos[max(cid[g], 0)];
+#pragma unroll
+        for (int k = 0; k < PP; ++k) {
+            const int i = b0 + t + k * T;
+            if (i >= b1) break;
+            double2 f = rps[i + 1 - b0] > rps[i - b0] ? fv[k] : make_double2(0.0, 0.0);
+            f.x -= rv[k].x;  // f - calcRepForce(x_i), :1059
+            f.y -= rv[k].y;
             const double mag = sqrt(f.x * f.x + f.y * f.y);
             double2 q = bpos[i - b0];
             q.x = q.x + (f.x / mag) * a.step;

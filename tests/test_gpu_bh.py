@@ -149,22 +149,30 @@ def _hubby_graph(n, hubs, seed):
     return graphs.csr_from_edges(n, e)
 
 
-@pytest.mark.parametrize("kind,algo", [("grid", 4), ("hubs", 4), ("hubs", 40)])
-def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo):
-    # batches of 2048 vertices carry 8k..40k CSR entries: the walk spreads them over several
-    # workgroups; it must agree with the one-workgroup walk up to the order of partial sums
+@pytest.mark.parametrize("kind,algo,weighted,batch", [("grid", 4, False, 2048), ("hubs", 4, False, 2048),
+                                                      ("hubs", 40, False, 2048), ("hubs", 4, True, 1000),
+                                                      ("dense", 4, True, 2048), ("dense", 40, False, 1500)])
+def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo, weighted, batch):
+    # batches of 1000..2048 vertices carry 8k..130k CSR entries: the walk spreads them over
+    # several workgroups (on "dense" a thread sums more entries than it loads a batch ahead);
+    # it must agree with the one-workgroup walk up to the order of partial sums
     if kind == "grid":
         rp, ci = graphs.csr_from_edges(*graphs.grid(100, 100))
-    else:
+    elif kind == "hubs":
         rp, ci = _hubby_graph(12000, 4, 3)
+    else:
+        rng = np.random.default_rng(5)
+        e = rng.integers(0, 12000, size=(12000 * 30, 2))
+        rp, ci = graphs.csr_from_edges(12000, e[e[:, 0] != e[:, 1]])
     n = len(rp) - 1
+    w = np.random.default_rng(4).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
     x0 = np.random.default_rng(2).random((n, 2))
-    print(f"[case] grid walk {kind} algo {algo} n {n} nnz {len(ci)}", flush=True)
+    print(f"[case] grid walk {kind} algo {algo} weighted {weighted} batch {batch} n {n} nnz {len(ci)}", flush=True)
     out = {}
     for mode in ("single", "grid"):
         monkeypatch.setenv("BL_WALK", mode)
-        with bl.Graph(rp, ci) as g:
-            out[mode], st = g.run(x0, algo=algo, iters=2, batch=2048, bht=1.2)
+        with bl.Graph(rp, ci, w) as g:
+            out[mode], st = g.run(x0, algo=algo, iters=2, batch=batch, bht=1.2)
             out[mode + "_E"] = st.energy
     d = diameter(out["single"])
     assert np.abs(out["grid"] - out["single"]).max() <= 1e-10 * d
