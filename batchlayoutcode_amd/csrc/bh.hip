@@ -49,8 +49,15 @@ int launch_energy(const double *eterm, int n, double *out, int precision, hipStr
 namespace {
 
 constexpr int WG = 256;
-constexpr int SORT_ITEMS = 16;                 // keys per thread per sort tile
-constexpr int SORT_TILE = WG * SORT_ITEMS;     // 4096 keys per workgroup
+// Keys per thread per sort tile (tile = WG x rounds keys): at least 4, and enough that the
+// 256 x tiles histogram counts fit the one-launch scan (32,768 counts) up to 512k keys;
+// beyond that 16 (the three-launch scan).  Smaller tiles give more workgroups and fewer
+// serial rounds each (measured on 250k keys: 16 rounds per tile, 62 workgroups, took 16 us
+// per scatter pass; 4 rounds 7.4 us).
+inline int sort_rounds(int n) {
+    const int r = (n + 32767) / 32768;
+    return r <= 16 ? std::max(4, r) : 16;
+}
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = WG * SCAN_ITEMS;     // 2048 entries per workgroup
 constexpr int STACK = 128;                     // per-wave traversal stack (depth <= 32 -> <= 100 entries)
@@ -220,47 +227,64 @@ __global__ __launch_bounds__(WG) void k_scan_apply(const int *__restrict__ in, i
     }
 }
 
-// Exclusive scan of up to SCAN_SINGLE_MAX ints by one workgroup: tiles of 1024 x 4, each
-// thread scans its 4 in order, the thread sums are scanned across the workgroup (waves by
-// shuffles, then the wave totals), and a carry runs from tile to tile.  *total = the sum.
-constexpr int SCAN_SINGLE_MAX = 32768;
+// Exclusive scan of up to SCAN_SINGLE_MAX ints by one workgroup, in 8 tiles of 1024 x 4:
+// thread t holds items 4t..4t+3 of every tile.  All 8 tiles are loaded at once, coalesced
+// and unconditionally (the input buffer holds at least SCAN_SINGLE_MAX ints; items at or past
+// n are masked), the thread sums of every tile are scanned over the wave by shuffles, and
+// one barrier later every thread adds the wave totals before it.  *total = the sum.
+constexpr int SCAN_SINGLE_TILES = 8;
+constexpr int SCAN_SINGLE_MAX = SCAN_SINGLE_TILES * 4096;
+
+__device__ __forceinline__ int4 scan_item4(const int4 *src, int q, int i, int n) {
+    int4 v = src[q * 1024];
+    v.x = i < n ? v.x : 0;
+    v.y = i + 1 < n ? v.y : 0;
+    v.z = i + 2 < n ? v.z : 0;
+    v.w = i + 3 < n ? v.w : 0;
+    return v;
+}
 
 __global__ __launch_bounds__(1024) void k_scan_single(const int *__restrict__ in, int n, int *__restrict__ out,
                                                       int *__restrict__ total) {
-    __shared__ int wsum[16];
+    __shared__ int wsum[SCAN_SINGLE_TILES][16];  // [tile][wave]
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    int carry = 0;
-    for (int base = 0; base < n; base += 4096) {
-        int v[4];
-        int a = 0;
+    const int4 *src = reinterpret_cast<const int4 *>(in) + t;
+    int4 v[SCAN_SINGLE_TILES];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int i = base + 4 * t + k;
-            v[k] = i < n ? in[i] : 0;
-            a += v[k];
-        }
-        // inclusive scan of a over the wave
+    for (int q = 0; q < SCAN_SINGLE_TILES; ++q) v[q] = scan_item4(src, q, q * 4096 + 4 * t, n);
+    int ex[SCAN_SINGLE_TILES];  // exclusive prefix of the thread's items within the tile, over the wave
+#pragma unroll
+    for (int q = 0; q < SCAN_SINGLE_TILES; ++q) {
+        const int a = v[q].x + v[q].y + v[q].z + v[q].w;
         int x = a;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const int y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
-        if (lane == 63) wsum[w] = x;
-        __syncthreads();
-        int before = carry;
-        for (int q = 0; q < w; ++q) before += wsum[q];
-        int tile = 0;
-        for (int q = 0; q < 16; ++q) tile += wsum[q];
-        int run = before + x - a;  // exclusive prefix of this thread's first item
+        if (lane == 63) wsum[q][w] = x;
+        ex[q] = x - a;
+    }
+    __syncthreads();
+    int carry = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int i = base + 4 * t + k;
-            if (i < n) out[i] = run;
-            run += v[k];
+    for (int q = 0; q < SCAN_SINGLE_TILES; ++q) {
+        int before = 0, tile = 0;
+        for (int p = 0; p < 16; ++p) {
+            const int sp = wsum[q][p];
+            tile += sp;
+            before += p < w ? sp : 0;
+        }
+        const int i = q * 4096 + 4 * t;
+        const int r0 = carry + before + ex[q], r1 = r0 + v[q].x, r2 = r1 + v[q].y, r3 = r2 + v[q].z;
+        if (i + 3 < n) {
+            *reinterpret_cast<int4 *>(out + i) = make_int4(r0, r1, r2, r3);
+        } else {
+            if (i < n) out[i] = r0;
+            if (i + 1 < n) out[i + 1] = r1;
+            if (i + 2 < n) out[i + 2] = r2;
         }
         carry += tile;
-        __syncthreads();
     }
     if (t == 0) *total = carry;
 }
@@ -268,13 +292,13 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int *__restrict__ in
 // ---------------------------------------------------------------------------------------
 // 3. LSD radix sort, 8-bit digits.  hist is digit-major: hist[d * nblk + blk].
 __global__ __launch_bounds__(WG) void k_sort_hist(const uint64_t *__restrict__ keys, int n, int shift, int nblk,
-                                                  int *__restrict__ hist) {
+                                                  int rounds, int *__restrict__ hist) {
     __shared__ int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * SORT_TILE;
+    const int base = blockIdx.x * WG * rounds;
 #pragma unroll 4
-    for (int r = 0; r < SORT_ITEMS; ++r) {
+    for (int r = 0; r < rounds; ++r) {
         const int i = base + r * WG + threadIdx.x;
         if (i < n) atomicAdd(&h[(int)((keys[i] >> shift) & 255u)], 1);
     }
@@ -286,7 +310,7 @@ __global__ __launch_bounds__(WG) void k_sort_hist(const uint64_t *__restrict__ k
 // ballots to find the lanes that share a digit and per-wave counts in LDS.
 __global__ __launch_bounds__(WG) void k_sort_scatter(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, int *__restrict__ vout, int n,
-                                                     int shift, int nblk, const int *__restrict__ offs) {
+                                                     int shift, int nblk, int rounds, const int *__restrict__ offs) {
     __shared__ int running[256];
     __shared__ int wcount[WG / 64][256];
     __shared__ int base_off[256];
@@ -294,8 +318,8 @@ __global__ __launch_bounds__(WG) void k_sort_scatter(const uint64_t *__restrict_
     running[tid] = 0;
     base_off[tid] = offs[tid * nblk + blockIdx.x];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int base = blockIdx.x * SORT_TILE;
-    for (int r = 0; r < SORT_ITEMS; ++r) {
+    const int base = blockIdx.x * WG * rounds;
+    for (int r = 0; r < rounds; ++r) {
 #pragma unroll
         for (int q = 0; q < WG / 64; ++q) wcount[q][tid] = 0;
         __syncthreads();
@@ -1742,12 +1766,14 @@ int scan_excl(const int *in, int n, int *out, int *bsum, hipStream_t s) {
 int ensure_bh(BHWork &w, int n) {
     if (w.n_alloc >= n) return BL_OK;
     const size_t N = (size_t)n;
-    const int nblk = (n + SORT_TILE - 1) / SORT_TILE;
+    // every size up to n: the tiles are never smaller than WG x 4 keys
+    const int nblk = (n + WG * 4 - 1) / (WG * 4);
     const int nscan = std::max(256 * nblk, n);
     const int nsb = (nscan + SCAN_TILE - 1) / SCAN_TILE + 1;
     if (w.keys_a.ensure(8 * N) || w.keys_b.ensure(8 * N) || w.idx_a.ensure(4 * N) || w.idx_b.ensure(4 * N) ||
-        w.hist.ensure(4 * (size_t)256 * nblk) || w.scan_tmp.ensure(4 * (size_t)(256 * nblk + 1)) ||
-        w.spos.ensure(16 * N) || w.frame.ensure(sizeof(double4) * 1024 + 64) || w.isrep.ensure(4 * N) ||
+        w.hist.ensure(4 * std::max((size_t)256 * nblk, (size_t)SCAN_SINGLE_MAX)) ||
+        w.scan_tmp.ensure(4 * (size_t)(256 * nblk + 1)) || w.spos.ensure(16 * N) ||
+        w.frame.ensure(sizeof(double4) * 1024 + 64) || w.isrep.ensure(4 * std::max(N, (size_t)SCAN_SINGLE_MAX)) ||
         w.nodeid.ensure(4 * N) || w.node_child.ensure(sizeof(BHNode) * N) || w.rep.ensure(16 * N) ||
         w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_cxy.ensure(16 * N) || w.node_cnt.ensure(4 * N))
         return BL_E_NOMEM;
@@ -1771,16 +1797,17 @@ int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *
     int *ia = w.idx_a.as<int>(), *ib = w.idx_b.as<int>();
     const int g1 = (n + WG - 1) / WG;
     hipLaunchKernelGGL(k_keys, dim3(g1), dim3(WG), 0, s, d_pos, n, frame, ka, ia, d_xs, d_ys);
-    const int nblk = (n + SORT_TILE - 1) / SORT_TILE;
+    const int rounds = sort_rounds(n);
+    const int nblk = (n + WG * rounds - 1) / (WG * rounds);
     int *hist = w.hist.as<int>();
     int *offs = w.scan_tmp.as<int>();
     int *bsum = misc + 64;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(WG), 0, s, ka, n, shift, nblk, hist);
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(WG), 0, s, ka, n, shift, nblk, rounds, hist);
         rc = scan_excl(hist, 256 * nblk, offs, bsum, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(WG), 0, s, ka, ia, kb, ib, n, shift, nblk, offs);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(WG), 0, s, ka, ia, kb, ib, n, shift, nblk, rounds, offs);
         std::swap(ka, kb);
         std::swap(ia, ib);
     }

@@ -22,25 +22,35 @@ constexpr int MLP_W1 = 0, MLP_B1 = 512, MLP_W2 = 576, MLP_B2 = 4672, MLP_W3 = 47
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16, round to nearest even (inputs are finite)
-__device__ __forceinline__ short f2bf(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32; inputs are finite)
+__device__ __forceinline__ short f2bf(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ bf16x4 pack4(f32x4 v) {
-    bf16x4 r;
-    r[0] = f2bf(v[0]);
-    r[1] = f2bf(v[1]);
-    r[2] = f2bf(v[2]);
-    r[3] = f2bf(v[3]);
-    return r;
+// K = 32 step (gfx950's v_mfma_f32_16x16x32_bf16, twice the work of the K = 16 one per
+// issue) over the k-blocks 2h and 2h+1 of 16: a lane's 8 operands are its 4 of block 2h
+// followed by its 4 of block 2h+1, in A and B alike.  The contraction does not depend on
+// the order of k, so this gives the same sum as two K = 16 steps without moving activations
+// between lanes.
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+    return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 pack4(f32x4 v) {  // two v_cvt_pk_bf16_f32
+    typedef __bf16 hb4 __attribute__((ext_vector_type(4)));
+    hb4 h;
+    h[0] = (__bf16)v[0];
+    h[1] = (__bf16)v[1];
+    h[2] = (__bf16)v[2];
+    h[3] = (__bf16)v[3];
+    return __builtin_bit_cast(bf16x4, h);
 }
 
 // A fragment of a row-major BF16 matrix M (row stride ld): rows r0..r0+15, k0..k0+15;
@@ -49,8 +59,6 @@ __device__ __forceinline__ bf16x4 frag_a(const short *M, int ld, int r0, int k0,
     return *reinterpret_cast<const bf16x4 *>(M + (r0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4));
 }
 
-// GELU, tanh form: g(z) = z/2 (1 + tanh(c (z + 0.044715 z^3))), c = sqrt(2/pi).
-// tanh(u) = 1 - 2/(exp(2u) + 1) with the hardware exp2 and reciprocal.
 // GELU in tanh form, 0.5 z (1 + tanh(u)) with u = sqrt(2/pi) (z + 0.044715 z^3), evaluated as
 // z - z / (1 + e) with e = exp(2u) = 2^(z (GA + GB z^2)): 0.5 (1 + tanh u) = 1 - 1/(1 + e).
 // Six VALU operations and two transcendentals; z -> -inf gives e -> 0 and exactly 0, z -> inf
@@ -119,8 +127,14 @@ __device__ __forceinline__ void mlp_hidden(const short *W, int ld, const float *
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
             f32x4 acc = bias;
+            if constexpr (KB % 2 == 0) {
 #pragma unroll
-            for (int kb = 0; kb < KB; ++kb) acc = mfma16(a[kb], Hin[kb][sb], acc);
+                for (int h = 0; h < KB / 2; ++h)
+                    acc = mfma32(cat8(a[2 * h], a[2 * h + 1]), cat8(Hin[2 * h][sb], Hin[2 * h + 1][sb]), acc);
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) acc = mfma16(a[kb], Hin[kb][sb], acc);
+            }
             Z[ob][sb] = acc;
             f32x4 h;
 #pragma unroll
@@ -143,7 +157,7 @@ __device__ __forceinline__ void mlp_out(const MlpLds &L, const bf16x4 (&H)[4][4]
     for (int sb = 0; sb < 4; ++sb) {
         f32x4 acc = bias;
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) acc = mfma16(a[kb], H[kb][sb], acc);
+        for (int h = 0; h < 2; ++h) acc = mfma32(cat8(a[2 * h], a[2 * h + 1]), cat8(H[2 * h][sb], H[2 * h + 1][sb]), acc);
         Y[sb] = acc;
     }
 }
