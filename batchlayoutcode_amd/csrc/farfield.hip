@@ -217,7 +217,7 @@ __device__ void far_flush(const MlpLds &L, FarQueue &Q, int count, int lane, uin
                           double &fy) {
     auto feat = [&](int s, int f) { return s < count ? Q.feat[s][f] : 0.0f; };
     f32x4 Y[4];
-    mlp_forward(L, feat, Y, lane);
+    mlp_forward<false>(L, feat, Y, lane);  // K = 16: fewer registers, higher occupancy
     if (lane < 16) {
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {

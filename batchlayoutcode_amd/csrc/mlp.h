@@ -112,8 +112,9 @@ __device__ __forceinline__ void mlp_load(MlpLds &L, const float *__restrict__ P,
 
 // One hidden layer for the wave's 64 samples: Z[ob][sb] = W (64 x 16*KB) H[kb][sb] + b, and
 // the activations gelu(Z) as the next layer's B fragments.  Z is returned (FP32) when the
-// caller needs it for the backward pass.
-template <int KB>
+// caller needs it for the backward pass.  K32 selects the K = 32 MFMA for even KB: fewer
+// issues, but more registers (the learned traversal, whose occupancy they set, uses K = 16).
+template <int KB, bool K32 = true>
 __device__ __forceinline__ void mlp_hidden(const short *W, int ld, const float *b, const bf16x4 (&Hin)[4][4],
                                            f32x4 (&Z)[4][4], bf16x4 (&Hout)[4][4], int lane) {
 #pragma unroll
@@ -127,7 +128,7 @@ __device__ __forceinline__ void mlp_hidden(const short *W, int ld, const float *
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
             f32x4 acc = bias;
-            if constexpr (KB % 2 == 0) {
+            if constexpr (K32 && KB % 2 == 0) {
 #pragma unroll
                 for (int h = 0; h < KB / 2; ++h)
                     acc = mfma32(cat8(a[2 * h], a[2 * h + 1]), cat8(Hin[2 * h][sb], Hin[2 * h + 1][sb]), acc);
@@ -146,6 +147,7 @@ __device__ __forceinline__ void mlp_hidden(const short *W, int ld, const float *
 
 // Output layer: Y[sb] = W4 H3[.][sb] + b4; lanes 0..15 hold (y0, y1) of sample sb*16 + lane
 // in elements 0 and 1.
+template <bool K32 = true>
 __device__ __forceinline__ void mlp_out(const MlpLds &L, const bf16x4 (&H)[4][4], f32x4 (&Y)[4], int lane) {
     bf16x4 a[4];
 #pragma unroll
@@ -156,8 +158,14 @@ __device__ __forceinline__ void mlp_out(const MlpLds &L, const bf16x4 (&H)[4][4]
 #pragma unroll
     for (int sb = 0; sb < 4; ++sb) {
         f32x4 acc = bias;
+        if constexpr (K32) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) acc = mfma32(cat8(a[2 * h], a[2 * h + 1]), cat8(H[2 * h][sb], H[2 * h + 1][sb]), acc);
+            for (int h = 0; h < 2; ++h)
+                acc = mfma32(cat8(a[2 * h], a[2 * h + 1]), cat8(H[2 * h][sb], H[2 * h + 1][sb]), acc);
+        } else {
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb) acc = mfma16(a[kb], H[kb][sb], acc);
+        }
         Y[sb] = acc;
     }
 }
@@ -183,15 +191,15 @@ __device__ __forceinline__ void mlp_input(F feat, bf16x4 (&X)[4][4], int lane) {
 }
 
 // Full forward pass of the wave's 64 samples (inference).
-template <class F>
+template <bool K32 = true, class F>
 __device__ __forceinline__ void mlp_forward(const MlpLds &L, F feat, f32x4 (&Y)[4], int lane) {
     bf16x4 X[4][4], H1[4][4], H2[4][4];
     f32x4 Z[4][4];
     mlp_input(feat, X, lane);
-    mlp_hidden<1>(L.w1, 16, L.b1, X, Z, H1, lane);
-    mlp_hidden<4>(L.w2, MLP_H, L.b2, H1, Z, H2, lane);
-    mlp_hidden<4>(L.w3, MLP_H, L.b3, H2, Z, H1, lane);
-    mlp_out(L, H1, Y, lane);
+    mlp_hidden<1, K32>(L.w1, 16, L.b1, X, Z, H1, lane);
+    mlp_hidden<4, K32>(L.w2, MLP_H, L.b2, H1, Z, H2, lane);
+    mlp_hidden<4, K32>(L.w3, MLP_H, L.b3, H2, Z, H1, lane);
+    mlp_out<K32>(L, H1, Y, lane);
 }
 
 }  // namespace bl
