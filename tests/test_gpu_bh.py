@@ -62,6 +62,21 @@ def test_duplicate_points_are_an_error(bl):
     assert ei.value.code == -5
 
 
+@pytest.mark.parametrize("n", [131072, 131073, 524288, 524289])
+def test_sort_tile_and_scan_boundaries(bl, oracle, n):
+    # the radix sort's tile size follows n, and its histogram counts go to the one-workgroup
+    # scan up to 32,768 counts (n = 131072 and 524288 land exactly there) and to the
+    # three-launch scan beyond (n = 524289): permutation and FP64 forces stay bit-exact
+    pts = np.random.default_rng(n).random((n, 2))
+    print(f"[case] sort boundaries n {n}", flush=True)
+    t = bl.bh_tree(pts)
+    ref = oracle.bh_tree(pts)
+    assert np.array_equal(t["perm"], ref["perm"])
+    q = pts[np.random.default_rng(1).choice(n, 64, replace=False)]
+    got = bl.bh_forces(pts, 1.2, queries=q, precision=bl.PREC_FP64)
+    assert np.array_equal(got, oracle.bh_forces(pts, 1.2, queries=q))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5])
 def test_tiny_point_sets(bl, oracle, n):
     pts = np.random.default_rng(n).random((n, 2))
