@@ -175,7 +175,8 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
                      &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model, &g->cpre, &g->cpv,
-                     &g->early_rp, &g->early_ci, &g->early_w, &g->late_rp, &g->late_ci, &g->late_w};
+                     &g->early_rp, &g->early_ci, &g->early_w, &g->late_rp, &g->late_ci, &g->late_w,
+                     &g->grid_rows};
     for (DevBuf *b : all) b->release();
     g->bh.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);

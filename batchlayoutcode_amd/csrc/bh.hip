@@ -1423,6 +1423,7 @@ constexpr int GW_MIN_ENTRIES = 4096;  // mean CSR entries per batch from which i
 
 struct GridWalk {
     const int *rowptr, *colids;
+    const int *rows;  // [batch][GW_G][GW_T + 2]: each thread's first row, then the share's first and last
     const float *w_edge;
     double W;
     double2 *pos;
@@ -1468,6 +1469,19 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
     int cid[GW_K];
     float wf[GW_K];
     double2 pj[GW_K];
+    // the row of this thread's first entry and the share's first and last rows (host table),
+    // loaded a batch ahead as well; the share's rows as vector loads of values equal in all
+    // lanes (a uniform load would become a scalar load, whose wait counter LDS shares)
+    int fr, frf, frl;
+    auto load_rows = [&](int bb) {
+        const size_t rb = ((size_t)bb * G + w) * (T + 2);
+        int vb;
+        __asm__("v_mov_b32 %0, %1" : "=v"(vb) : "s"(0));
+        fr = a.rows[rb + t];
+        frf = a.rows[rb + T + vb];
+        frl = a.rows[rb + T + 1 + vb];
+    };
+    load_rows(0);
     // prologue: batch 0
     {
         const int cc = min(n, B);
@@ -1510,14 +1524,6 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
             nE0 = a.rowptr[va];
             nE1 = a.rowptr[ve];
         }
-        auto row_of = [&](int entry) {  // last r in [b0, b1) with rowptr[r] <= entry
-            int lo = b0, hi = b1;
-            while (hi - lo > 1) {
-                const int m = (lo + hi) >> 1;
-                if (rps[m - b0] <= entry) lo = m; else hi = m;
-            }
-            return lo;
-        };
         const int E0 = rps[0], E1 = rps[bc], total = E1 - E0;
         const int Cw = (total + G - 1) / G;
         const int Sw = min(E1, E0 + w * Cw), Ew = min(E1, Sw + Cw);
@@ -1527,7 +1533,7 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
         int hr = -1, tr = -1;
         double2 hv = make_double2(0.0, 0.0), tv = make_double2(0.0, 0.0);
         if (s < e) {
-            int r = row_of(s);
+            int r = fr;  // row_of(s)
             const int first = r;
             int rend = rps[r + 1 - b0];
             double2 pi = bpos[r - b0];
@@ -1610,7 +1616,7 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
         // A3. rows of this workgroup's share: complete ones -> fbuf, the first and last
         // (when they continue in other workgroups) -> whead / wtail
         if (Sw < Ew) {
-            const int rf = row_of(Sw), rl = row_of(Ew - 1);
+            const int rf = frf, rl = frl;  // row_of(Sw), row_of(Ew - 1)
             if (t == 0) {
                 a.wfirst[par * GW_G + w] = rf;
                 a.wlast[par * GW_G + w] = rl != rf ? rl : -1;
@@ -1715,6 +1721,7 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
         for (int k = 0; k < PP; ++k) nbp[k] = a.pos[min(b1 + t + k * T, n - 1)];
 #pragma unroll
         for (int g = 0; g < GW_K; ++g) pj[g] = a.pos[max(cid[g], 0)];
+        load_rows(min(b + 1, nb - 1));
 #pragma unroll
         for (int k = 0; k < PP; ++k) {
             const int i = b0 + t + k * T;
@@ -1861,13 +1868,52 @@ int *gwalk_err(BHWork &w) {
 
 // Launches one iteration's walk on GW_G co-resident workgroups; anything but BL_OK means
 // nothing was launched (the caller uses the one-workgroup walk).
+// The multi-workgroup walk's row lookup for batch size B (GridWalk::rows), built on the host
+// once per graph and batch size: the same shares as the kernel computes; for each thread the
+// last row r of the batch with rowptr[r] <= its first entry (-1 without entries), and the
+// rows of the share's first and last entries.
+int grid_rows_build(bl_graph *g, int B) {
+    if (g->grid_rows_batch == B) return BL_OK;
+    const int n = g->n, G = GW_G, T = GW_T, nb = (n + B - 1) / B;
+    const int32_t *rp = g->h_rowptr.data();
+    std::vector<int32_t> tab((size_t)nb * G * (T + 2));
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = b * B, b1 = std::min(n, b0 + B);
+        const int E0 = rp[b0], E1 = rp[b1];
+        const int Cw = (E1 - E0 + G - 1) / G, c = std::max(1, (Cw + T - 1) / T);
+        auto row_of = [&](int entry) { return (int)(std::upper_bound(rp + b0, rp + b1, entry) - rp) - 1; };
+        for (int w = 0; w < G; ++w) {
+            int32_t *o = &tab[((size_t)b * G + w) * (T + 2)];
+            const int Sw = std::min(E1, E0 + w * Cw), Ew = std::min(E1, Sw + Cw);
+            int r = b0;  // first entries rise with t: the row advances monotonically
+            for (int t = 0; t < T; ++t) {
+                const int s0 = std::min(Ew, Sw + t * c), e0 = std::min(Ew, s0 + c);
+                if (s0 >= e0) {
+                    o[t] = -1;
+                    continue;
+                }
+                while (r + 1 < b1 && rp[r + 1] <= s0) ++r;
+                o[t] = r;
+            }
+            o[T] = Sw < Ew ? row_of(Sw) : -1;
+            o[T + 1] = Sw < Ew ? row_of(Ew - 1) : -1;
+        }
+    }
+    if (g->grid_rows.ensure(tab.size() * sizeof(int32_t))) return BL_E_NOMEM;
+    BL_HIP(hipMemcpy(g->grid_rows.p, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    g->grid_rows_batch = B;
+    return BL_OK;
+}
+
 int launch_walk_grid(bl_graph *g, double2 *pos, const bl_params *p, double step, hipStream_t s) {
     BHWork &w = g->bh;
     if (w.gwalk.ensure(GW_BYTES)) return BL_E_NOMEM;
+    if (int rc = grid_rows_build(g, p->batch)) return rc;
     char *base = w.gwalk.as<char>();
     GridWalk a;
     a.rowptr = g->rowptr.as<int>();
     a.colids = g->colids.as<int>();
+    a.rows = g->grid_rows.as<int>();
     a.w_edge = g->has_w ? g->w_edge.as<float>() : nullptr;
     a.W = p->weight;
     a.pos = pos;

@@ -69,6 +69,11 @@ struct bl_graph {
     // batch) and "late" ones (neighbour in the same or a later batch: its iteration-start
     // position), each in CSR order
     bl::DevBuf early_rp, early_ci, early_w, late_rp, late_ci, late_w;
+    // the multi-workgroup walk's row lookup at batch size grid_rows_batch: for every batch,
+    // workgroup and thread, the row of the thread's first entry, then the first and last
+    // rows of the workgroup's share
+    int32_t grid_rows_batch = 0;
+    bl::DevBuf grid_rows;
     // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),
     // as a CSR of strictly increasing column ids per row (+ their weights)
     bl::DevBuf eff_rowptr, eff_colids, eff_w;
