@@ -50,13 +50,13 @@ namespace {
 
 constexpr int WG = 256;
 // Keys per thread per sort tile (tile = WG x rounds keys): at least 4, and enough that the
-// 256 x tiles histogram counts fit the one-launch scan (32,768 counts) up to 512k keys;
+// 256 x tiles histogram counts fit the one-launch scan (32,768 counts) up to 1M keys;
 // beyond that 16 (the three-launch scan).  Smaller tiles give more workgroups and fewer
 // serial rounds each (scatter pass on 250k keys: 16 rounds, 62 workgroups, 16 us; 8 rounds,
 // 122 workgroups, 10 us; 4 rounds 7.4 us, but 62k counts then need the three-launch scan).
 inline int sort_rounds(int n) {
     const int r = (n + 32767) / 32768;
-    return r <= 16 ? std::max(4, r) : 16;
+    return r <= 32 ? std::max(4, r) : 16;
 }
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = WG * SCAN_ITEMS;     // 2048 entries per workgroup

@@ -62,11 +62,11 @@ def test_duplicate_points_are_an_error(bl):
     assert ei.value.code == -5
 
 
-@pytest.mark.parametrize("n", [131072, 131073, 524288, 524289])
+@pytest.mark.parametrize("n", [131072, 131073, 1048576, 1048577])
 def test_sort_tile_and_scan_boundaries(bl, oracle, n):
     # the radix sort's tile size follows n, and its histogram counts go to the one-workgroup
-    # scan up to 32,768 counts (n = 131072 and 524288 land exactly there) and to the
-    # three-launch scan beyond (n = 524289): permutation and FP64 forces stay bit-exact
+    # scan up to 32,768 counts (n = 131072 and 1048576 land exactly there) and to the
+    # three-launch scan beyond (n = 1048577): permutation and FP64 forces stay bit-exact
     pts = np.random.default_rng(n).random((n, 2))
     print(f"[case] sort boundaries n {n}", flush=True)
     t = bl.bh_tree(pts)
