@@ -1442,6 +1442,27 @@ struct GridWalk {
 
 __device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
+// One step of a segmented inclusive scan over the wave by a DPP move: each lane takes the
+// value and key of the lane CTRL names (row_shr:n within rows of 16 lanes; row_bcast:15 /
+// row_bcast:31, the last lane of the previous row / of the first two rows, into the rows
+// of ROW_MASK) and adds the value when the keys are equal.  Lanes without a source lane
+// see the key -2, which no lane holds.  After the steps 1, 2, 4, 8, bcast15, bcast31 lane l
+// holds the sum of the values of its run (the lanes before it with its key) up to l:
+// runs are contiguous, so a run that reaches back into an earlier row of 16 contains that
+// row's last lane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_step(double2 &v, int key) {
+    const int ks = __builtin_amdgcn_update_dpp(-2, key, CTRL, ROW_MASK, 0xF, false);
+    const int xl = __builtin_amdgcn_update_dpp(0, __double2loint(v.x), CTRL, ROW_MASK, 0xF, false);
+    const int xh = __builtin_amdgcn_update_dpp(0, __double2hiint(v.x), CTRL, ROW_MASK, 0xF, false);
+    const int yl = __builtin_amdgcn_update_dpp(0, __double2loint(v.y), CTRL, ROW_MASK, 0xF, false);
+    const int yh = __builtin_amdgcn_update_dpp(0, __double2hiint(v.y), CTRL, ROW_MASK, 0xF, false);
+    if (ks == key) {
+        v.x += __hiloint2double(xh, xl);
+        v.y += __hiloint2double(yh, yl);
+    }
+}
+
 __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
     constexpr int T = GW_T;
     constexpr int PR = (WALK_MAXB + T) / T;      // rowptr values per thread (a batch has bc + 1)
@@ -1588,17 +1609,14 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
         head[t] = hv;
         tail[t] = tv;
         // A2. segmented inclusive scan of the first-row partials over the threads (runs of
-        // equal hrow are contiguous): in the wave by shuffles, then across waves
+        // equal hrow are contiguous): in the wave by DPP moves, then across waves
         double2 v = hv;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const double px = __shfl_up(v.x, off, 64), py = __shfl_up(v.y, off, 64);
-            const int kp = __shfl_up(hr, off, 64);
-            if (lane >= off && kp == hr) {
-                v.x += px;
-                v.y += py;
-            }
-        }
+        seg_step<0x111, 0xF>(v, hr);  // row_shr:1
+        seg_step<0x112, 0xF>(v, hr);  // row_shr:2
+        seg_step<0x114, 0xF>(v, hr);  // row_shr:4
+        seg_step<0x118, 0xF>(v, hr);  // row_shr:8
+        seg_step<0x142, 0xA>(v, hr);  // row_bcast:15 into rows 1 and 3
+        seg_step<0x143, 0xC>(v, hr);  // row_bcast:31 into rows 2 and 3
         if (lane == 63) {
             wsum[wv] = v;
             wklast[wv] = hr;
