@@ -17,10 +17,29 @@ namespace bl {
 // reference's `dist > 0` test skips them.
 constexpr float kTiny = 1e-30f;
 
+// v + the value of the lane CTRL names (a DPP move; lanes without a source lane, or outside
+// ROW_MASK, add +0.0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+    return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of the wave (all active), the same value in every lane: an inclusive
+// scan by DPP moves (row_shr 1, 2, 4, 8 within rows of 16 lanes, then row_bcast:15 into rows
+// 1 and 3 and row_bcast:31 into rows 2 and 3) leaves the total in lane 63, which is
+// broadcast.  DPP moves cost a few cycles where the LDS permutes of a shuffle butterfly
+// cost a round trip through the LDS unit each.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v = dpp_add<0x111, 0xF>(v);
+    v = dpp_add<0x112, 0xF>(v);
+    v = dpp_add<0x114, 0xF>(v);
+    v = dpp_add<0x118, 0xF>(v);
+    v = dpp_add<0x142, 0xA>(v);
+    v = dpp_add<0x143, 0xC>(v);
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
 // Attraction of one neighbour pair, in the reference's FP64 operation order:
