@@ -362,10 +362,21 @@ __global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part,
     const int slot = (int)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (slot >= bc) return;
+    const int i = b0 + slot;
     const double2 *row = part + (size_t)slot * P;
-    const int pe = G + (cpre[b0 + slot + 1] - cpre[b0 + slot]);
+    // Loads that wait for nothing are issued together: the vertex's position, its number of
+    // correction pieces, and the G chunk partials, which every row has whatever that number;
+    // the pieces follow in the same per-thread order (p = tid, tid + WG, ...).
+    const double2 x0 = pos[i];
+    const int pe = G + (cpre[i + 1] - cpre[i]);
     double sx = 0.0, sy = 0.0;
-    for (int p = tid; p < pe; p += WG) {
+    int p = tid;
+    for (; p < G; p += WG) {
+        const double2 v = row[p];
+        sx += v.x;
+        sy += v.y;
+    }
+    for (; p < pe; p += WG) {
         const double2 v = row[p];
         sx += v.x;
         sy += v.y;
@@ -377,7 +388,8 @@ __global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part,
     if (tid == 0) {
         const double fx = (red[0].x + red[1].x) + (red[2].x + red[3].x);
         const double fy = (red[0].y + red[1].y) + (red[2].y + red[3].y);
-        move_one(pos, b0 + slot, fx, fy, step, eterm);
+        pos[i] = moved(x0, fx, fy, step);
+        eterm[i] = fx * fx + fy * fy;
     }
 }
 
