@@ -1890,9 +1890,14 @@ int *gwalk_err(BHWork &w) {
 // once per graph and batch size: the same shares as the kernel computes; for each thread the
 // last row r of the batch with rowptr[r] <= its first entry (-1 without entries), and the
 // rows of the share's first and last entries.
+// Past GW_ROWS_MAX_BYTES (about 11,000 batches) it is not built and the run takes the
+// one-workgroup walk instead.
+constexpr size_t GW_ROWS_MAX_BYTES = size_t(1) << 30;
+
 int grid_rows_build(bl_graph *g, int B) {
     if (g->grid_rows_batch == B) return BL_OK;
     const int n = g->n, G = GW_G, T = GW_T, nb = (n + B - 1) / B;
+    if ((size_t)nb * G * (T + 2) * sizeof(int32_t) > GW_ROWS_MAX_BYTES) return BL_E_NOMEM;
     const int32_t *rp = g->h_rowptr.data();
     std::vector<int32_t> tab((size_t)nb * G * (T + 2));
     for (int b = 0; b < nb; ++b) {
