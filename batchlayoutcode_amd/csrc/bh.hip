@@ -1774,6 +1774,9 @@ struct TreeBufs {
     double *frame;
 };
 
+// Exclusive scan of in[0, n) into out, the total at bsum[nb].  `in` must hold at least
+// SCAN_SINGLE_MAX ints whatever n is (k_scan_single reads all its tiles; ensure_bh sizes the
+// histogram and boundary-flag buffers so).
 int scan_excl(const int *in, int n, int *out, int *bsum, hipStream_t s) {
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (n <= SCAN_SINGLE_MAX) {  // one launch instead of three
