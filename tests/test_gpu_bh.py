@@ -183,15 +183,21 @@ def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo, weight
     w = np.random.default_rng(4).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
     x0 = np.random.default_rng(2).random((n, 2))
     print(f"[case] grid walk {kind} algo {algo} weighted {weighted} batch {batch} n {n} nnz {len(ci)}", flush=True)
-    out = {}
-    for mode in ("single", "grid"):
-        monkeypatch.setenv("BL_WALK", mode)
-        with bl.Graph(rp, ci, w) as g:
-            out[mode], st = g.run(x0, algo=algo, iters=2, batch=batch, bht=1.2)
-            out[mode + "_E"] = st.energy
-    d = diameter(out["single"])
-    assert np.abs(out["grid"] - out["single"]).max() <= 1e-10 * d
-    assert abs(out["grid_E"] - out["single_E"]) <= 1e-10 * out["single_E"]
+    # iteration 1 from x0, then iteration 2 teacher-forced from the one-workgroup walk's
+    # iteration-1 layout: free-running, the order of the partial sums alone can grow to 1e-8
+    # by iteration 2 on the dense graph (a vertex whose force nearly cancels)
+    start = x0
+    for it in range(2):
+        out = {}
+        for mode in ("single", "grid"):
+            monkeypatch.setenv("BL_WALK", mode)
+            with bl.Graph(rp, ci, w) as g:
+                out[mode], st = g.run(start, algo=algo, iters=1, batch=batch, bht=1.2, step0=0.999 ** it)
+                out[mode + "_E"] = st.energy
+        d = diameter(out["single"])
+        assert np.abs(out["grid"] - out["single"]).max() <= 1e-10 * d, it
+        assert abs(out["grid_E"] - out["single_E"]) <= 1e-10 * out["single_E"], it
+        start = out["single"]
 
 
 @pytest.mark.parametrize("kind,weighted,batch,algo,prec", [("rgg", False, 256, 4, 0), ("rgg", True, 256, 4, 0),
