@@ -1417,7 +1417,7 @@ __global__ __launch_bounds__(2 * WROW_T) void k_walk_row(const int *__restrict__
 // from the LDS copy), so that the next batch starts on data already in registers.
 // The results differ from the one-workgroup walk only in the order of the partial sums.
 constexpr int GW_T = 1024;
-constexpr int GW_G = 24;  // 8..32 measured on the R-MAT walk: 24 and 32 are fastest
+constexpr int GW_G = 24;  // 8..32 measured on the R-MAT walk: 24 is fastest
 constexpr int GW_K = 4;  // entries per thread loaded a batch ahead
 constexpr int GW_MIN_ENTRIES = 4096;  // mean CSR entries per batch from which it is used
 
