@@ -281,6 +281,11 @@ def main():
         if bh is not None:
             out["cpu_baseline_algo4"] = bh
             log(f"cpu baseline algo 4: {bh['value']:.3e} vertex-updates/s")
+            # and on one thread: the reference's tree sort asks for 32 threads whatever
+            # -threads says, which can make one thread the faster setting on small graphs
+            bh1 = cpu_baseline(n, rp, ci, init, B, 1, algo=4, iters=3)
+            out["cpu_baseline_algo4_1thread"] = bh1
+            log(f"cpu baseline algo 4, one thread: {bh1['value']:.3e} vertex-updates/s")
     print(json.dumps(out), flush=True)
 
 
