@@ -416,7 +416,11 @@ extern "C" int bl_run_batched(int32_t count, const int32_t *voff, const int64_t 
     if (rc) return rc;
     if (p->algo != BL_ALGO_CACHE && p->algo != BL_ALGO_FA) return BL_E_ARG;
     if (p->precision == BL_PREC_FP64 || p->converged) return BL_E_ARG;
-    if (voff[0] != 0) return BL_E_ARG;
+    if (voff[0] != 0 || eoff[0] != 0) return BL_E_ARG;
+    for (int32_t k = 0; k < count; ++k)  // offsets first, so the loop below reads only inside the arrays
+        if (voff[k + 1] <= voff[k] || voff[k + 1] - voff[k] > BL_BATCHED_MAX_N || eoff[k + 1] < eoff[k])
+            return BL_E_ARG;
+    if (eoff[count] > INT32_MAX || (eoff[count] > 0 && !colids)) return BL_E_ARG;
     for (int32_t k = 0; k < count; ++k) {  // per-graph CSR validation
         const int32_t n = voff[k + 1] - voff[k];
         if (n <= 0 || n > BL_BATCHED_MAX_N) return BL_E_ARG;

@@ -205,8 +205,9 @@ int bl_graph_set_far_model(bl_graph *g, const float *params);
 /* Lays out `count` independent graphs in one persistent launch (no counterpart in the
  * reference, which lays out one graph per process; each graph follows the same algorithm as
  * bl_run).  Graph k has vertices [voff[k], voff[k+1]) and CSR entries [eoff[k], eoff[k+1])
- * of the concatenated arrays; rowptr is global (rowptr[voff[k]] == eoff[k]) and colids are
- * local to the graph.  Every graph runs p->algo (2 or 8) with p->batch/iters/weight in
+ * of the concatenated arrays (voff[0] == eoff[0] == 0); rowptr is global
+ * (rowptr[voff[k]] == eoff[k]) and colids are local to the graph.  Offsets and CSR are
+ * validated on the host before any device work (BL_E_ARG / BL_E_GRAPH).  Every graph runs p->algo (2 or 8) with p->batch/iters/weight in
  * precision BL_PREC_MIXED or BL_PREC_FP32; energies[k] (nullable) receives each graph's final
  * energy and finish_ms[k] (nullable) the time from the launch start to the graph's
  * completion.  Graphs of up to BL_BATCHED_MAX_N vertices. */

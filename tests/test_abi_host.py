@@ -128,3 +128,63 @@ def test_output_write_is_byte_identical_to_reference(bl, fx, tmp_path):
 def test_params_default_is_reference_cli_default(bl):
     p = bl.params()
     assert (p.algo, p.iters, p.batch, p.bht, p.engt, p.weight, p.step0) == (2, 600, 256, 1.2, 0.01, 1.0, 1.0)
+
+
+E_ARG, E_GRAPH = -1, -7
+
+
+@pytest.mark.parametrize("rp,ci,code", [
+    ([1, 1, 2], [1, 0], E_GRAPH),          # rowptr[0] != 0
+    ([0, 2, 1, 2], [1, 0], E_GRAPH),       # rowptr not monotone
+    ([0, 1, 2], [1, 2], E_GRAPH),          # colid == n
+    ([0, 1, 2], [-1, 0], E_GRAPH),         # negative colid
+    ([0, 1, 3], [1, 0], E_GRAPH),          # rowptr[n] != nnz
+])
+def test_graph_create_rejects_bad_csr_before_device_work(bl, rp, ci, code):
+    """bl_graph_create validates the CSR on the host before it touches a device, so each of
+    these returns its error with no GPU present."""
+    import ctypes as C
+
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    h = C.c_void_p()
+    assert bl.lib().bl_graph_create(len(rp) - 1, len(ci), rp, ci.ctypes.data, None, C.byref(h)) == code
+    assert not h.value
+
+
+@pytest.mark.parametrize("case", ["eoff0", "voff_empty_graph", "eoff_decreasing", "rowptr_offset",
+                                  "colid_global", "rowptr_not_monotone", "algo4"])
+def test_run_batched_rejects_bad_input_before_device_work(bl, case):
+    """bl_run_batched checks its offsets, then each graph's CSR, on the host before any device
+    work: two triangles, each spoiled in one way, return BL_E_ARG or BL_E_GRAPH with no GPU."""
+    import ctypes as C
+
+    voff = np.array([0, 3, 6], np.int32)
+    eoff = np.array([0, 6, 12], np.int64)
+    rowptr = np.array([0, 2, 4, 6, 8, 10, 12], np.int32)
+    colids = np.array([1, 2, 0, 2, 0, 1] * 2, np.int32)
+    kw, code = {}, E_ARG
+    if case == "eoff0":
+        eoff[:] = [1, 7, 13]
+        rowptr += 1
+    elif case == "voff_empty_graph":
+        voff[1] = 0
+    elif case == "eoff_decreasing":
+        eoff[1] = 13
+    elif case == "rowptr_offset":
+        rowptr[3] = 5
+        code = E_GRAPH
+    elif case == "colid_global":
+        colids[7] = 4  # graph 1 has 3 vertices; colids are local to the graph
+        code = E_GRAPH
+    elif case == "rowptr_not_monotone":
+        rowptr[1] = 5
+        code = E_GRAPH
+    elif case == "algo4":
+        kw = dict(algo=4)
+    xy = np.zeros((6, 2))
+    p = bl.params(iters=1, batch=2, **kw)
+    st = bl.bl_stats()
+    rc = bl.lib().bl_run_batched(2, voff, eoff, rowptr, colids, xy, C.byref(p), np.zeros(2), np.zeros(2),
+                                 C.byref(st))
+    assert rc == code
