@@ -150,11 +150,17 @@ def _random_graph(rng, n, m, empty_rows=(), dup=False):
     return rp.astype(np.int32), cols.astype(np.int32)
 
 
-@pytest.mark.parametrize("case", ["isolated_rows", "duplicates", "weights", "b1", "b3", "b_gt_n", "selfloop"])
+@pytest.mark.parametrize("case", ["isolated_rows", "duplicates", "weights", "b1", "b3", "b_gt_n", "selfloop",
+                                  "edgeless", "n2"])
 def test_edge_cases_match_oracle(bl, oracle, case):
     rng = np.random.default_rng(11)
     n, B, w = 70, 16, None
-    if case == "isolated_rows":
+    if case == "edgeless":  # nnz == 0: repulsion only, every row empty
+        rp, ci = np.zeros(n + 1, np.int32), np.zeros(0, np.int32)
+    elif case == "n2":  # the smallest graph with a force: one edge, a batch wider than the graph
+        n = 2
+        rp, ci = np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32)
+    elif case == "isolated_rows":
         rp, ci = _random_graph(rng, n, 150, empty_rows=(5, 6, 40, n - 1))
     elif case == "duplicates":
         rp, ci = _random_graph(rng, n, 150, dup=True)
