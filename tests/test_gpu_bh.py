@@ -122,6 +122,32 @@ def test_algo4_fp32_first_and_teacher_forced(bl, fx):
             assert err <= 1e-4
 
 
+@pytest.mark.parametrize("case", ["edgeless", "n2", "n3_path"])
+def test_algo4_degenerate_graphs_match_oracle(bl, oracle, case):
+    """Barnes-Hut layouts of graphs with no edges (repulsion only) and of the smallest graphs:
+    the FP64 path bit-exact with the oracle over 3 iterations, the default path within 1e-4 of
+    the layout diameter after one."""
+    rng = np.random.default_rng(12)
+    if case == "edgeless":
+        n = 300
+        rp, ci = np.zeros(n + 1, np.int32), np.zeros(0, np.int32)
+    elif case == "n2":
+        n = 2
+        rp, ci = np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32)
+    else:
+        n = 3
+        rp, ci = np.array([0, 1, 3, 4], np.int32), np.array([1, 0, 2, 1], np.int32)
+    init = rng.random((n, 2))
+    ref, e_ref = oracle.bh_layout(rp, ci, init, 3, 16, theta=1.2)
+    ref1, _ = oracle.bh_layout(rp, ci, init, 1, 16, theta=1.2)
+    print(f"[case] algo 4 degenerate {case}", flush=True)
+    with bl.Graph(rp, ci) as g:
+        xy, st = g.run(init, algo=4, iters=3, batch=16, bht=1.2, precision=bl.PREC_FP64)
+        assert np.array_equal(xy, ref) and st.energy == e_ref
+        xy1, _ = g.run(init, algo=4, iters=1, batch=16, bht=1.2)
+    assert np.abs(xy1 - ref1).max() <= 1e-4 * diameter(ref1)
+
+
 def test_algo4_final_energy_600_iterations(bl):
     """600 iterations of algo 4 on the 64x64 grid from the fixture init and 17 1-ulp
     perturbations of it (the band fixture's scheme).  The trajectory is chaotic: two walks that
