@@ -41,6 +41,27 @@ def test_batched_matches_oracle(bl, oracle, algo, batch):
         assert abs(en[k] - e_ref) <= 1e-8 * e_ref
 
 
+@pytest.mark.parametrize("all_edgeless", [False, True])
+def test_batched_degenerate_graphs(bl, oracle, all_edgeless):
+    """Graphs with no edges, a two-vertex graph and a graph whose last rows are empty, laid out
+    in one launch beside an ordinary one; and a launch in which no graph has an edge."""
+    rng = np.random.default_rng(13)
+    gs = [(np.zeros(51, np.int32), np.zeros(0, np.int32)), (np.zeros(4, np.int32), np.zeros(0, np.int32))]
+    if not all_edgeless:
+        rp, ci = graphs.csr_from_edges(*graphs.grid(8, 8))
+        tail = np.concatenate([rp, np.full(5, rp[-1], rp.dtype)])  # five isolated vertices at the end
+        gs += [(np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32)), (tail, ci),
+               _mixed_graphs(rng, [100])[0]]
+    xys = [np.random.default_rng(300 + k).random((len(rp) - 1, 2)) for k, (rp, _) in enumerate(gs)]
+    print(f"[case] batched degenerate all_edgeless={all_edgeless}", flush=True)
+    out, en, fin, _ = bl.run_batched(gs, xys, algo=2, iters=3, batch=16)
+    for k, ((rp, ci), x0) in enumerate(zip(gs, xys)):
+        ref, e_ref, _ = oracle.minibatch(rp, ci, x0, 3, 16, law=2)
+        err = np.abs(out[k] - ref).max() / diameter(ref)
+        assert err <= 1e-8, (k, len(rp) - 1, err)
+        assert abs(en[k] - e_ref) <= 1e-8 * e_ref
+
+
 def test_batched_global_memory_path_and_fp32(bl, oracle):
     # graphs too large for the LDS copy read their positions from global memory
     rng = np.random.default_rng(9)
