@@ -24,6 +24,8 @@ FAR_MONO, FAR_EXACT, FAR_LEARNED, FAR_QUAD = 1, 2, 3, 4
 FAR_MODEL_PARAMS, FAR_FEATURES = 9026, 8
 PREC_MIXED, PREC_FP64, PREC_FP32 = 0, 1, 2
 BATCHED_MAX_N = 8192
+# the Barnes-Hut family's batch walks (Stats.walk)
+WALK_NONE, WALK_ROW_ORDER, WALK_PF, WALK_SPLIT, WALK_ROW, WALK_GRID = 0, 1, 2, 3, 4, 5
 
 
 class BLError(RuntimeError):
@@ -40,8 +42,9 @@ class bl_params(C.Structure):
 
 class bl_stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("iters_done", C.c_int32), ("status", C.c_int32),
-                ("seconds", C.c_double), ("step_final", C.c_double), ("kernel_ms", C.c_double * 4),
-                ("kernel_launches", C.c_int32 * 4)]
+                ("seconds", C.c_double), ("step_final", C.c_double), ("kernel_ms", C.c_double * 8),
+                ("kernel_launches", C.c_int32 * 8), ("walk", C.c_int32), ("far_cells", C.c_int64),
+                ("far_model_iters", C.c_int32), ("far_judge_err", C.c_float * 2)]
 
 
 _lib = None
@@ -111,13 +114,17 @@ class Stats:
     iters_done: int
     seconds: float
     step_final: float
-    kernel_ms: tuple = (0.0, 0.0, 0.0, 0.0)
-    kernel_launches: tuple = (0, 0, 0, 0)
+    kernel_ms: tuple = (0.0,) * 8
+    kernel_launches: tuple = (0,) * 8
+    walk: int = 0
+    far_cells: int = 0
+    far_model_iters: int = 0
+    far_judge_err: tuple = (0.0, 0.0)
 
 
 def _stats(st: bl_stats) -> Stats:
     return Stats(st.energy, st.iters_done, st.seconds, st.step_final, tuple(st.kernel_ms),
-                 tuple(st.kernel_launches))
+                 tuple(st.kernel_launches), st.walk, st.far_cells, st.far_model_iters, tuple(st.far_judge_err))
 
 
 class Graph:

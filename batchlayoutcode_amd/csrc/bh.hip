@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "bh_common.h"
@@ -557,7 +558,32 @@ struct Frame {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// FP32 mode.  One wave per 64 Morton-consecutive vertices; shared stack of (node, mask).
+// Unit vector d/|d| added to (fx, fy).  REFINE (the default precision): FP64 difference, the
+// FP32 reciprocal square root refined by one FP64 Newton step (relative error ~1e-14) and an
+// FP64 sum.  Otherwise (BL_PREC_FP32) all in FP32.  Over hundreds of iterations the FP32
+// terms bias the layout: the 600-iteration energy on the 64x64 grid ends several percent
+// below the reference's, while the refined terms agree with it.
+template <bool REFINE>
+struct UnitSum {
+    typename std::conditional<REFINE, double, float>::type x = 0, y = 0;
+    __device__ __forceinline__ void add(double dx, double dy) {
+        if constexpr (REFINE) {
+            const double r2 = fma(dy, dy, dx * dx);
+            double q = (double)__builtin_amdgcn_rsqf((float)r2);
+            q = q * fma(-0.5 * r2 * q, q, 1.5);
+            x = fma(dx, q, x);
+            y = fma(dy, q, y);
+        } else {
+            const float fx = (float)dx, fy = (float)dy;
+            const float inv = __builtin_amdgcn_rsqf(fmaf(fy, fy, fx * fx));
+            x = fmaf(fx, inv, x);
+            y = fmaf(fy, inv, y);
+        }
+    }
+};
+
+// FP32 / mixed modes.  One wave per 64 Morton-consecutive vertices; shared stack of (node, mask).
+template <bool REFINE>
 __global__ __launch_bounds__(WG) void k_rep32(const double2 *__restrict__ qpos, const int *__restrict__ qid, int n,
                                               const double2 *__restrict__ spos,
                                               const BHNode *__restrict__ nodes, const int *__restrict__ misc,
@@ -573,7 +599,7 @@ __global__ __launch_bounds__(WG) void k_rep32(const double2 *__restrict__ qpos, 
     const double maxmin = frame[2];
     const float th2 = (float)(theta * theta);
     const float lo_band = 1.0f - 1.0f / 4096.0f, hi_band = 1.0f + 1.0f / 4096.0f;
-    float fx = 0.f, fy = 0.f;
+    UnitSum<REFINE> f;
     const int root = misc[1];
     int sp = 0;
     uint64_t m0 = __ballot(valid);
@@ -605,11 +631,7 @@ __global__ __launch_bounds__(WG) void k_rep32(const double2 *__restrict__ qpos, 
         if (a < b * lo_band) acc = true;
         else if (a > b * hi_band) acc = false;
         else acc = D / sqrt(dx64 * dx64 + dy64 * dy64) < theta;  // exact FP64 test, :71-72
-        if (act && acc) {
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            fx = fmaf(dx, inv, fx);
-            fy = fmaf(dy, inv, fy);
-        }
+        if (act && acc) f.add(dx64, dy64);
         const uint64_t rej = __ballot(act && !acc);
         if (rej) {
             const int c[4] = {nd.ch.x, nd.ch.y, nd.ch.z, nd.ch.w};
@@ -624,20 +646,15 @@ __global__ __launch_bounds__(WG) void k_rep32(const double2 *__restrict__ qpos, 
                 } else if (c[q] <= -2) {
                     if ((rej >> lane) & 1ull) {  // leaf, :78-83
                         const double2 l = spos[-c[q] - 2];
-                        const float lx = (float)(l.x - p.x), ly = (float)(l.y - p.y);
-                        const float lr2 = fmaf(ly, ly, lx * lx);
-                        if (lr2 > 0.f) {
-                            const float inv = __builtin_amdgcn_rsqf(lr2);
-                            fx = fmaf(lx, inv, fx);
-                            fy = fmaf(ly, inv, fy);
-                        }
+                        const double lx = l.x - p.x, ly = l.y - p.y;
+                        if (lx * lx + ly * ly > 0.0) f.add(lx, ly);
                     }
                 }
             }
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if (valid) rep[qid[k]] = make_double2((double)fx, (double)fy);
+    if (valid) rep[qid[k]] = make_double2((double)f.x, (double)f.y);
 }
 
 // FP64 mode: the reference's depth-first order (quadrants 0..3, leaves in place) and its
@@ -1871,9 +1888,9 @@ int launch_rep(BHWork &w, int n, double theta, int precision, hipStream_t s) {
                            w.spos.as<double2>(), w.node_child.as<BHNode>(), w.misc.as<int>(), w.frame.as<double>(), theta,
                            w.rep.as<double2>(), w.misc.as<int>());
     else
-        hipLaunchKernelGGL(k_rep32, dim3(g), dim3(WG), 0, s, w.spos.as<double2>(), w.idx_a.as<int>(), n,
-                           w.spos.as<double2>(), w.node_child.as<BHNode>(), w.misc.as<int>(), w.frame.as<double>(), theta,
-                           w.rep.as<double2>(), w.misc.as<int>());
+        hipLaunchKernelGGL(precision == BL_PREC_FP32 ? k_rep32<false> : k_rep32<true>, dim3(g), dim3(WG), 0, s,
+                           w.spos.as<double2>(), w.idx_a.as<int>(), n, w.spos.as<double2>(), w.node_child.as<BHNode>(),
+                           w.misc.as<int>(), w.frame.as<double>(), theta, w.rep.as<double2>(), w.misc.as<int>());
     BL_HIP(hipGetLastError());
     return BL_OK;
 }
@@ -2013,6 +2030,18 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb));
     }
     g->bh.gwalk_used = 0;
+    int walk = BL_WALK_NONE;
+    // cells accepted by the far-field traversal, counted per iteration in misc[8..9]
+    unsigned long long *ncells = reinterpret_cast<unsigned long long *>(g->bh.misc.as<int>() + 8);
+    // the learned far field is judged every judge_every iterations from the first (BL_FAR_JUDGE=k
+    // sets the period; 0 keeps the model on without judging)
+    int judge_every = 10;
+    if (const char *je = getenv("BL_FAR_JUDGE")) judge_every = atoi(je);
+    const bool judged = far_mode == FAR_LEARNED && judge_every > 0 && n >= 2;
+    if (judged) {
+        rc = far_judge_reset(g->bh, s);
+        if (rc) return rc;
+    }
     // the exact mode builds no tree, so nothing else clears the status word the walk reads
     if (far_mode == FAR_EXACT) BL_HIP(hipMemsetAsync(g->bh.misc.p, 0, 16, s));
     for (int it = 0; it < iters; ++it) {
@@ -2022,37 +2051,60 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
             prof.end();
             if (rc) return rc;
         }
+        const float *model = g->has_far_model ? g->far_model.as<float>() : nullptr;
+        if (far_mode && far_mode != FAR_EXACT) {
+            prof.begin(4);
+            rc = far_cells_of(g->bh, n, s);
+            prof.end();
+            if (rc) return rc;
+            BL_HIP(hipMemsetAsync(ncells, 0, sizeof(unsigned long long), s));
+        }
+        if (judged && it % judge_every == 0) {
+            prof.begin(6);
+            rc = far_judge(g->bh, pos, n, theta, model, it, s);
+            prof.end();
+            if (rc) return rc;
+        }
         prof.begin(0);
-        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, far_mode, g->has_far_model ? g->far_model.as<float>() : nullptr, s)
+        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, far_mode, model, s, ncells, judged)
                       : launch_rep(g->bh, n, theta, p->precision, s);
         prof.end();
         if (rc) return rc;
+        if (p->precision != BL_PREC_FP64 && !use_grid && use_row) {
+            // the row walk's pre-pass: attraction of the neighbours in the same or a later batch
+            prof.begin(5);
+            hipLaunchKernelGGL(k_late_attract, dim3((n + 255) / 256), dim3(256), 0, s, g->late_rp.as<int>(),
+                               g->late_ci.as<int>(), g->has_w ? g->late_w.as<float>() : nullptr, p->weight, pos,
+                               g->bh.rep.as<double2>(), n);
+            prof.end();
+        }
         prof.begin(1);
-        if (p->precision == BL_PREC_FP64)
+        if (p->precision == BL_PREC_FP64) {
+            walk = BL_WALK_ROW_ORDER;
             hipLaunchKernelGGL(k_attract_move, dim3(1), dim3(1024), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
                                g->bh.misc.as<int>());
-        else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK)
-            ;
-        else if (use_row) {
-            hipLaunchKernelGGL(k_late_attract, dim3((n + 255) / 256), dim3(256), 0, s, g->late_rp.as<int>(),
-                               g->late_ci.as<int>(), g->has_w ? g->late_w.as<float>() : nullptr, p->weight, pos,
-                               g->bh.rep.as<double2>(), n);
+        } else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK) {
+            walk = BL_WALK_GRID;
+        } else if (use_row) {
+            walk = BL_WALK_ROW;
             hipLaunchKernelGGL(g->has_w ? k_walk_row<true> : k_walk_row<false>, dim3(1), dim3(2 * WROW_T),
                                wrow_lds(g->has_w), s, g->early_rp.as<int>(), g->early_ci.as<int>(),
                                g->has_w ? g->early_w.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                n, p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
-        }
-        else if (p->batch <= WALK_MAXB)
+        } else if (p->batch <= WALK_MAXB) {
+            walk = BL_WALK_PF;
             hipLaunchKernelGGL(k_walk_pf, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(), n,
                                p->batch, STEP, g->eterm.as<double>(), g->bh.misc.as<int>());
-        else
+        } else {
+            walk = BL_WALK_SPLIT;
             hipLaunchKernelGGL(k_walk<false>, dim3(1), dim3(WALK_T), 0, s, g->rowptr.as<int>(), g->colids.as<int>(),
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
                                g->bh.misc.as<int>());
+        }
         prof.end();
         prof.begin(2);
         rc = launch_energy(g->eterm.as<double>(), n, g->energy.as<double>(), p->precision, s);
@@ -2062,6 +2114,16 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         STEP = STEP * 0.999;  // :1070
     }
     prof.report(st);
+    if (st) st->walk = walk;
+    unsigned long long cells = 0;
+    if (far_mode && far_mode != FAR_EXACT && iters > 0)
+        BL_HIP(hipMemcpyAsync(&cells, ncells, sizeof(cells), hipMemcpyDeviceToHost, s));
+    int model_iters = far_mode == FAR_LEARNED ? iters : 0;
+    float judge_err[2] = {0.0f, 0.0f};
+    if (judged && iters > 0) {
+        rc = far_judge_read(g->bh, &model_iters, judge_err, s);
+        if (rc) return rc;
+    }
     int status = 0;
     double E = 1.79769313486231570815e+308;  // ENERGY starts at DBL_MAX (:1024)
     if (iters > 0) {
@@ -2074,6 +2136,10 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     if (status & ST_DUPKEY) return BL_E_DUPKEY;
     if (gerr) return BL_E_DEVICE;  // a grid barrier timed out (the walk stopped early)
     if (st) {
+        st->far_cells = (int64_t)cells;
+        st->far_model_iters = model_iters;
+        st->far_judge_err[0] = judge_err[0];
+        st->far_judge_err[1] = judge_err[1];
         st->energy = E;
         st->iters_done = iters;
         st->step_final = STEP;
@@ -2179,8 +2245,9 @@ int bh_forces_host(int32_t n, const double *xy, double theta, int32_t nq, const 
         hipLaunchKernelGGL(k_rep64, dim3(gq), dim3(WG), 0, sc.s, dq, qid, nq, w.spos.as<double2>(), w.node_child.as<BHNode>(),
                            w.misc.as<int>(), w.frame.as<double>(), theta, dout, w.misc.as<int>());
     else
-        hipLaunchKernelGGL(k_rep32, dim3(gq), dim3(WG), 0, sc.s, dq, qid, nq, w.spos.as<double2>(), w.node_child.as<BHNode>(),
-                           w.misc.as<int>(), w.frame.as<double>(), theta, dout, w.misc.as<int>());
+        hipLaunchKernelGGL(precision == BL_PREC_FP32 ? k_rep32<false> : k_rep32<true>, dim3(gq), dim3(WG), 0, sc.s, dq,
+                           qid, nq, w.spos.as<double2>(), w.node_child.as<BHNode>(), w.misc.as<int>(),
+                           w.frame.as<double>(), theta, dout, w.misc.as<int>());
     BL_HIP(hipGetLastError());
     int status = 0;
     BL_HIP(hipMemcpyAsync(&status, w.misc.p, sizeof(int), hipMemcpyDeviceToHost, sc.s));

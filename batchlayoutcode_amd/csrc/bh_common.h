@@ -29,8 +29,20 @@ int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 // w.rep (by vertex id).  FAR_EXACT sums over all points; the other modes read the tree of
 // d_pos already built in w by bh_build_tree.  FAR_LEARNED needs model (device, MLP_NPARAM
 // floats).
+// The other modes need the cell summaries of that tree first (far_cells_of).  ncells
+// (device, nullable) is incremented by the number of cells the traversal accepts.
 enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3, FAR_QUAD = 4 };
+int far_cells_of(BHWork &w, int n, hipStream_t s);
+// judged (FAR_LEARNED): the traversal follows the verdict of the last far_judge (the model, or
+// the analytic second-order correction when the model lost)
 int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode, const float *model,
-                  hipStream_t s);
+                  hipStream_t s, unsigned long long *ncells, bool judged);
+// The learned field's judge: the exact repulsion at a sample of the positions d_pos (of the tree
+// in w), against the learned and the analytic second-order fields there; sets the verdict on
+// the device.  far_judge_reset starts a run (model on, counters cleared); far_judge_read queues
+// the copies of the number of traversals that used the model and the last medians.
+int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float *model, int it, hipStream_t s);
+int far_judge_reset(BHWork &w, hipStream_t s);
+int far_judge_read(BHWork &w, int *model_iters, float *err, hipStream_t s);
 
 }  // namespace bl

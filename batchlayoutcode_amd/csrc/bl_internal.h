@@ -43,6 +43,7 @@ struct BHWork {
     DevBuf rep;                              // repulsion result per vertex (double2)
     DevBuf misc;                             // counters / status words
     DevBuf far_cells, far_arrive;            // physically consistent far field: cell summaries
+    DevBuf judge;                            // learned far field: the judge's sample, partials, flag
     DevBuf gwalk;                            // multi-workgroup walk: partials, barrier, error flag
     int gwalk_used = 0;                      // the last run launched it
     int32_t n_alloc = 0;
@@ -100,8 +101,8 @@ struct Prof {
     std::vector<hipEvent_t> ev;
     std::vector<int> cls;
     size_t used = 0;
-    double ms[4] = {0, 0, 0, 0};
-    int cnt[4] = {0, 0, 0, 0};
+    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     void begin(int c) {
         if (!on) return;
         if (2 * (used + 1) > ev.size()) {
@@ -153,7 +154,7 @@ struct Prof {
     void report(bl_stats *st) {
         flush();
         if (!st) return;
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < 8; ++c) {
             st->kernel_ms[c] += ms[c];
             st->kernel_launches[c] += cnt[c];
         }

@@ -246,7 +246,8 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
                                                  const BHNode *__restrict__ nodes, const FarCell *__restrict__ cells,
                                                  const int *__restrict__ misc, const double *__restrict__ frame,
                                                  double theta, const float *__restrict__ model,
-                                                 double2 *__restrict__ out) {
+                                                 double2 *__restrict__ out, unsigned long long *ncells,
+                                                 const int *use_model, int *model_iters) {
     __shared__ int stk_node[WG / 64][STACK];
     __shared__ uint64_t stk_mask[WG / 64][STACK];
     // learned mode: the model and the per-wave queues in dynamic LDS (LEARNED_LDS bytes)
@@ -254,7 +255,13 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
     MlpLds *L = reinterpret_cast<MlpLds *>(dyn);
     FarQueue *Qs = reinterpret_cast<FarQueue *>(dyn + QOFF);
     if (misc[0]) return;
-    if constexpr (MODE == TRAV_LEARNED) {
+    // learned mode: the judge's verdict (use_model, nullable: always the model); when it rejected
+    // the model the cells get the analytic second-order correction instead
+    const bool learned = MODE == TRAV_LEARNED && (use_model == nullptr || *use_model != 0);
+    const bool quad = MODE == TRAV_QUAD || (MODE == TRAV_LEARNED && !learned);
+    if (MODE == TRAV_LEARNED && model_iters && learned && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(model_iters, 1);
+    if (learned) {
         mlp_load(*L, model, threadIdx.x, WG);
         __syncthreads();
     }
@@ -276,6 +283,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         return;
     }
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    unsigned accepted = 0;  // cells this wave's lanes accepted (wave-uniform)
     int qn = 0;  // queued cells of this wave (learned mode)
     uint64_t mlo = 0, mhi = 0;  // this lane's queue slots 0..63 and 64..127 (learned mode)
     if (lane == 0) {
@@ -296,12 +304,13 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         const double r2 = dx * dx + dy * dy;
         const double D = ldexp(maxmin, -nd.depth);
         const bool acc = D * D < th2 * r2;
+        if (ncells) accepted += (unsigned)__popcll(__ballot(act && acc));
         if (act && acc) {
             const double s = (double)cl.m * inv_r2<true>(dx, dy);
             const double mx = dx * s, my = dy * s;
             fx += mx;
             fy += my;
-            if constexpr (MODE == TRAV_QUAD) {
+            if (quad) {
                 float f[8];
                 far_features(cl, dx, dy, r2, D, f);
                 const double a = (double)f[4], b = -2.0 * (double)f[2];
@@ -309,7 +318,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
                 fy += a * my + b * mx;
             }
         }
-        if constexpr (MODE == TRAV_LEARNED) {
+        if (MODE == TRAV_LEARNED && learned) {
             const uint64_t accm = __ballot(act && acc);
             if (accm) {
                 if (act && acc) {
@@ -365,10 +374,11 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if constexpr (MODE == TRAV_LEARNED) {
+    if (MODE == TRAV_LEARNED && learned) {
         if (qn > 0) far_flush(*L, Qs[w], qn, lane, mlo, fx, fy);
     }
     if (valid) out[qid[k]] = make_double2(fx, fy);
+    if (ncells && lane == 0) atomicAdd(ncells, (unsigned long long)accepted);
 }
 
 // Training samples: for query k, its accepted cells (at most cap) with their features and
@@ -474,30 +484,9 @@ __global__ __launch_bounds__(WG) void k_far_exact(const double2 *__restrict__ qp
     if (k < nq) out[qid ? qid[k] : k] = make_double2(-ax, -ay);  // pair_rep sums -d/|d|^2
 }
 
-int ensure_far(BHWork &w, int n) {
-    if (w.far_cells.ensure(sizeof(FarCell) * (size_t)std::max(n, 1)) || w.far_arrive.ensure(4 * (size_t)std::max(n, 1)))
-        return BL_E_NOMEM;
-    return BL_OK;
-}
-
-// cell summaries of the tree already built in w
-int far_cells_of(BHWork &w, int n, hipStream_t s) {
-    int rc = ensure_far(w, n);
-    if (rc) return rc;
-    if (n >= 2) {
-        BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
-        hipLaunchKernelGGL(k_far_small, dim3((n + 63) / 64), dim3(64), 0, s, w.node_child.as<BHNode>(),
-                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>());
-        hipLaunchKernelGGL(k_far_top, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
-                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>(),
-                           w.far_arrive.as<int>());
-        BL_HIP(hipGetLastError());
-    }
-    return BL_OK;
-}
-
 int launch_trav(BHWork &w, const double2 *qpos, const int *qid, int nq, double theta, int mode, const float *model,
-                double2 *out, hipStream_t s) {
+                double2 *out, hipStream_t s, unsigned long long *ncells = nullptr, const int *use_model = nullptr,
+                int *model_iters = nullptr) {
     const int g64 = ((nq + 63) / 64 + WG / 64 - 1) / (WG / 64);
     const BHNode *nodes = w.node_child.as<BHNode>();
     const FarCell *cells = w.far_cells.as<FarCell>();
@@ -506,13 +495,13 @@ int launch_trav(BHWork &w, const double2 *qpos, const int *qid, int nq, double t
     const double *frame = w.frame.as<double>();
     if (mode == FAR_MONO)
         hipLaunchKernelGGL(k_far_trav<TRAV_MONO>, dim3(g64), dim3(WG), 0, s, qpos, qid, nq, spos, nodes, cells, misc,
-                           frame, theta, model, out);
+                           frame, theta, model, out, ncells, use_model, model_iters);
     else if (mode == FAR_QUAD)
         hipLaunchKernelGGL(k_far_trav<TRAV_QUAD>, dim3(g64), dim3(WG), 0, s, qpos, qid, nq, spos, nodes, cells, misc,
-                           frame, theta, model, out);
+                           frame, theta, model, out, ncells, use_model, model_iters);
     else if (mode == FAR_LEARNED && model)
         hipLaunchKernelGGL(k_far_trav<TRAV_LEARNED>, dim3(g64), dim3(WG), LEARNED_LDS, s, qpos, qid, nq, spos, nodes, cells,
-                           misc, frame, theta, model, out);
+                           misc, frame, theta, model, out, ncells, use_model, model_iters);
     else
         return BL_E_ARG;
     BL_HIP(hipGetLastError());
@@ -538,10 +527,142 @@ struct HostRun {
     }
 };
 
+// ---- the learned far field's judge -------------------------------------------------------
+// Every few iterations the learned field is compared with the exact repulsion on a sample of
+// JUDGE_S vertices at the iteration-start positions, next to the analytic second-order field;
+// the traversals of the iteration then use the model only while its median relative force
+// error on the sample is at most the analytic term's.
+constexpr int JUDGE_S = 256;   // sample vertices
+constexpr int JUDGE_P = 256;   // j chunks of the exact sum
+
+// layout of BHWork::judge
+struct JudgeBuf {
+    double2 q[JUDGE_S];            // sample positions
+    int qid[JUDGE_S];              // 0..S-1 (output slots of the traversals)
+    double2 part[JUDGE_S * JUDGE_P];
+    double2 field[2 * JUDGE_S];    // learned, then analytic, at the sample
+    int use_model;                 // the verdict read by the traversals
+    int model_iters;               // traversals that used the model
+    float err[2];                  // median relative error of the learned and the analytic field
+};
+
+__global__ __launch_bounds__(JUDGE_S) void k_judge_sample(const double2 *__restrict__ pos, int n, int S, int offset,
+                                                           JudgeBuf *J) {
+    const int k = threadIdx.x;
+    if (k >= S) return;
+    const long long id = (long long)k * n / S + offset;
+    J->q[k] = pos[id < n ? id : n - 1];
+    J->qid[k] = k;
+}
+
+// part[k * P + c] = sum over points of chunk c of -(x_j - q_k)/|x_j - q_k|^2 (FP64-grade terms)
+__global__ __launch_bounds__(JUDGE_S) void k_judge_exact(const double2 *__restrict__ pts, int n, int S,
+                                                          JudgeBuf *__restrict__ J) {
+    typedef const __attribute__((address_space(4))) double cdbl;
+    const int k = threadIdx.x, c = blockIdx.x;
+    const int ch = (n + JUDGE_P - 1) / JUDGE_P;
+    const int j0 = min(n, c * ch), j1 = min(n, j0 + ch);
+    const double2 q = J->q[k < S ? k : 0];
+    cdbl *cp = (cdbl *)pts;
+    double ax = 0.0, ay = 0.0;
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pair_rep<true>(cp[2 * (j + u)], cp[2 * (j + u) + 1], q.x, q.y, ax, ay);
+    }
+    for (; j < j1; ++j) pair_rep<true>(cp[2 * j], cp[2 * j + 1], q.x, q.y, ax, ay);
+    if (k < S) J->part[k * JUDGE_P + c] = make_double2(ax, ay);
+}
+
+// Medians by rank (ties broken by index) and the verdict.
+__global__ __launch_bounds__(JUDGE_S) void k_judge_decide(int S, JudgeBuf *J) {
+    __shared__ float e[2][JUDGE_S];
+    const int k = threadIdx.x;
+    if (k < S) {
+        double ex = 0.0, ey = 0.0;
+        for (int c = 0; c < JUDGE_P; ++c) {
+            ex -= J->part[k * JUDGE_P + c].x;
+            ey -= J->part[k * JUDGE_P + c].y;
+        }
+        const double m = sqrt(ex * ex + ey * ey);
+        for (int t = 0; t < 2; ++t) {
+            const double2 f = J->field[t * JUDGE_S + k];
+            const double dx = f.x - ex, dy = f.y - ey;
+            e[t][k] = m > 0.0 ? (float)(sqrt(dx * dx + dy * dy) / m) : 0.0f;
+        }
+    }
+    __syncthreads();
+    if (k < S) {
+        for (int t = 0; t < 2; ++t) {
+            int rank = 0;
+            for (int j = 0; j < S; ++j) rank += (e[t][j] < e[t][k]) || (e[t][j] == e[t][k] && j < k);
+            if (rank == S / 2) J->err[t] = e[t][k];
+        }
+    }
+    __syncthreads();
+    if (k == 0) J->use_model = J->err[0] <= J->err[1] ? 1 : 0;
+}
+
 }  // namespace
 
+static int ensure_far(BHWork &w, int n) {
+    if (w.far_cells.ensure(sizeof(FarCell) * (size_t)std::max(n, 1)) || w.far_arrive.ensure(4 * (size_t)std::max(n, 1)))
+        return BL_E_NOMEM;
+    return BL_OK;
+}
+
+// cell summaries of the tree already built in w
+int far_cells_of(BHWork &w, int n, hipStream_t s) {
+    int rc = ensure_far(w, n);
+    if (rc) return rc;
+    if (n >= 2) {
+        BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
+        hipLaunchKernelGGL(k_far_small, dim3((n + 63) / 64), dim3(64), 0, s, w.node_child.as<BHNode>(),
+                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>());
+        hipLaunchKernelGGL(k_far_top, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
+                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>(),
+                           w.far_arrive.as<int>());
+        BL_HIP(hipGetLastError());
+    }
+    return BL_OK;
+}
+
+int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float *model, int it, hipStream_t s) {
+    if (w.judge.ensure(sizeof(JudgeBuf))) return BL_E_NOMEM;
+    JudgeBuf *J = w.judge.as<JudgeBuf>();
+    const int S = std::min(JUDGE_S, n);
+    const int stride = std::max(1, n / S);
+    const int offset = (int)(((long long)it * 7919) % stride);
+    hipLaunchKernelGGL(k_judge_sample, dim3(1), dim3(JUDGE_S), 0, s, d_pos, n, S, offset, J);
+    hipLaunchKernelGGL(k_judge_exact, dim3(JUDGE_P), dim3(JUDGE_S), 0, s, d_pos, n, S, J);
+    BL_HIP(hipGetLastError());
+    int rc = launch_trav(w, J->q, J->qid, S, theta, FAR_LEARNED, model, J->field, s);
+    if (rc) return rc;
+    rc = launch_trav(w, J->q, J->qid, S, theta, FAR_QUAD, model, J->field + JUDGE_S, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_judge_decide, dim3(1), dim3(JUDGE_S), 0, s, S, J);
+    BL_HIP(hipGetLastError());
+    return BL_OK;
+}
+
+int far_judge_reset(BHWork &w, hipStream_t s) {
+    if (w.judge.ensure(sizeof(JudgeBuf))) return BL_E_NOMEM;
+    JudgeBuf *J = w.judge.as<JudgeBuf>();
+    const int init[2] = {1, 0};  // use the model until the first verdict; no traversal counted yet
+    BL_HIP(hipMemcpyAsync(&J->use_model, init, sizeof(init), hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemsetAsync(J->err, 0, sizeof(J->err), s));
+    return BL_OK;
+}
+
+int far_judge_read(BHWork &w, int *model_iters, float *err, hipStream_t s) {
+    JudgeBuf *J = w.judge.as<JudgeBuf>();
+    BL_HIP(hipMemcpyAsync(model_iters, &J->model_iters, sizeof(int), hipMemcpyDeviceToHost, s));
+    BL_HIP(hipMemcpyAsync(err, J->err, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    return BL_OK;
+}
+
 int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode, const float *model,
-                  hipStream_t s) {
+                  hipStream_t s, unsigned long long *ncells, bool judged) {
     if (mode == FAR_EXACT) {
         // by vertex id, straight from the positions
         hipLaunchKernelGGL(k_far_exact, dim3((n + WG - 1) / WG), dim3(WG), 0, s, d_pos, nullptr, n, d_pos, n,
@@ -549,10 +670,11 @@ int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode
         BL_HIP(hipGetLastError());
         return BL_OK;
     }
-    // the tree of d_pos is already in w (bh_build_tree); queries in Morton order
-    int rc = far_cells_of(w, n, s);
-    if (rc) return rc;
-    return launch_trav(w, w.spos.as<double2>(), w.idx_a.as<int>(), n, theta, mode, model, w.rep.as<double2>(), s);
+    // the tree of d_pos and its cell summaries (far_cells_of) are already in w; queries in
+    // Morton order
+    JudgeBuf *J = judged ? w.judge.as<JudgeBuf>() : nullptr;
+    return launch_trav(w, w.spos.as<double2>(), w.idx_a.as<int>(), n, theta, mode, model, w.rep.as<double2>(), s,
+                       ncells, J ? &J->use_model : nullptr, J ? &J->model_iters : nullptr);
 }
 
 // Host entry: far-field forces of nq query points against the n points xy.

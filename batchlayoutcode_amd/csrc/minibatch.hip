@@ -20,14 +20,6 @@
 //     chunk workgroups write each batch vertex's chunk partials; its correction waves take
 //     the neighbour corrections in pieces of at most CORR_PIECE entries (a hub's
 //     correction is spread over many waves); k_move32 adds them in a fixed order and moves.
-//   * k_iter32 (BL_PERSISTENT=1, mixed precision): one persistent launch per iteration.
-//     Worker workgroups own a j chunk held in LDS for the whole iteration and compute,
-//     batch after batch, the chunk partial of every batch vertex; reducer workgroups form
-//     each batch vertex's correction, sum its partials in a fixed order and move it.  Steps
-//     are ordered by striped atomic counters; every wait has a deadline, on expiry the
-//     launch sets an error flag and all workgroups leave.  Correct but slower than the
-//     per-step launches: each of its many agent-scope hand-offs writes back or invalidates
-//     an XCD's L2.
 //
 // FP64 mode (reference-grade): the reference's own operations, in its order.  The
 // j range is cut into the reference's 512-wide blocks (blocky, :717); each block's
@@ -60,184 +52,6 @@ typedef const __attribute__((address_space(4))) double cdouble;
 __device__ __forceinline__ void move_one(double2 *pos, int i, double fx, double fy, double step, double *eterm) {
     pos[i] = moved(pos[i], fx, fy, step);
     eterm[i] = fx * fx + fy * fy;
-}
-
-// ---------------------------------------------------------------------------------------
-// Persistent per-iteration kernel (FP32 mode)
-constexpr int NSTRIPE = 32;   // counters are striped over 32 cache lines
-constexpr int CSTRIDE = 32;   // uint32 between stripes (128 B)
-constexpr int MAX_SLOTS = 64; // batch slots per reducer workgroup
-
-struct IterArgs {
-    double2 *pos;
-    int n, B, nb, NW, NR, chunk, Bmax;
-    double step;
-    double2 *part;            // [2][Bmax][NW]
-    const int *erp, *eci;
-    const float *ew;
-    double W;
-    int law;
-    double *eterm;
-    unsigned *ctr;            // A stripes, then B stripes, then C stripes
-    int *err;
-    long long ticks;          // wait deadline in wall-clock ticks
-};
-
-__device__ __forceinline__ unsigned stripe_members(int total, int k) {
-    return (unsigned)(total / NSTRIPE + (k < total % NSTRIPE ? 1 : 0));
-}
-
-// Waits until every stripe k of ctr holds stripe_members(total, k) * rounds (acquire).
-// Returns false when the deadline passed or another workgroup flagged an error.
-__device__ bool wait_rounds(const unsigned *ctr, int total, unsigned rounds, int *err, long long ticks, int *sflag) {
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        const unsigned target = lane < NSTRIPE ? stripe_members(total, lane) * rounds : 0u;
-        const long long t0 = wall_clock64();
-        int bad = 0;
-        for (;;) {
-            unsigned v = 0xffffffffu;
-            if (lane < NSTRIPE) v = __hip_atomic_load(ctr + lane * CSTRIDE, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (__all(v >= target)) break;
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                bad = 1;
-                break;
-            }
-            if (wall_clock64() - t0 > ticks) {
-                if (lane == 0) atomicOr(err, 1);
-                bad = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        if (lane == 0) *sflag = bad;
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return *sflag == 0;
-}
-
-// Publishes this workgroup's writes and bumps its stripe of ctr (release).
-__device__ void signal_done(unsigned *ctr, int who) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_fetch_add(ctr + (who % NSTRIPE) * CSTRIDE, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void dense_lds(const double2 *lds, int u0, int u1, double px, double py, double &ax,
-                                          double &ay) {
-    int u = u0;
-    for (; u + UNROLL <= u1; u += UNROLL) {
-#pragma unroll
-        for (int k = 0; k < UNROLL; ++k) {
-            const double2 q = lds[u + k];
-            pair_rep<true>(q.x, q.y, px, py, ax, ay);
-        }
-    }
-    for (; u < u1; ++u) {
-        const double2 q = lds[u];
-        pair_rep<true>(q.x, q.y, px, py, ax, ay);
-    }
-}
-
-__global__ __launch_bounds__(WG) void k_iter32(IterArgs a) {
-    extern __shared__ double2 lds[];   // worker: the chunk's positions
-    __shared__ double2 red[WG / 64];
-    __shared__ double2 corr[MAX_SLOTS];
-    __shared__ int sflag;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    unsigned *ctrA = a.ctr, *ctrB = a.ctr + NSTRIPE * CSTRIDE, *ctrC = a.ctr + 2 * NSTRIPE * CSTRIDE;
-    const int w = blockIdx.x;
-    if (w < a.NW) {
-        // ---- worker: chunk [j0, j1) resident in LDS --------------------------------------
-        const int j0 = w * a.chunk, j1 = min(a.n, j0 + a.chunk);
-        const int cnt = max(0, j1 - j0);
-        for (int k = tid; k < cnt; k += WG) lds[k] = a.pos[j0 + k];
-        __syncthreads();
-        for (int b = 0; b < a.nb; ++b) {
-            const int b0 = b * a.B, bc = min(a.B, a.n - b0);
-            const int buf = b & 1;
-            // the reducers of step b-2 have read this partial buffer
-            if (b >= 2 && !wait_rounds(ctrB, a.NR, b - 1, a.err, a.ticks, &sflag)) return;
-            // part of the chunk moved by the previous batch: processed last, after its moves
-            int ov0 = j1, ov1 = j1;
-            if (b >= 1) {
-                ov0 = max(j0, b0 - a.B);
-                ov1 = min(j1, b0);
-                if (ov0 >= ov1) ov0 = ov1 = j1;
-            }
-            const int rounds = (bc + WG - 1) / WG;
-            for (int r = 0; r < rounds; ++r) {
-                const int slot = r * WG + tid;
-                const bool act = slot < bc;
-                const double2 pi = a.pos[b0 + (act ? slot : 0)];
-                double ax = 0.0, ay = 0.0;
-                dense_lds(lds, 0, ov0 - j0, pi.x, pi.y, ax, ay);
-                dense_lds(lds, ov1 - j0, cnt, pi.x, pi.y, ax, ay);
-                if (ov0 < ov1) {
-                    if (r == 0) {
-                        if (!wait_rounds(ctrB, a.NR, b, a.err, a.ticks, &sflag)) return;
-                        for (int k = ov0 + tid; k < ov1; k += WG) lds[k - j0] = a.pos[k];
-                        __syncthreads();
-                    }
-                    dense_lds(lds, ov0 - j0, ov1 - j0, pi.x, pi.y, ax, ay);
-                }
-                if (act) a.part[((size_t)buf * a.Bmax + slot) * a.NW + w] = make_double2(ax, ay);
-            }
-            signal_done(ctrA, w);
-        }
-        return;
-    }
-    // ---- reducer r: batch slots r, r+NR, ... -------------------------------------------------
-    const int r = w - a.NW;
-    for (int b = 0; b < a.nb; ++b) {
-        const int b0 = b * a.B, bc = min(a.B, a.n - b0);
-        const int buf = b & 1;
-        // every reducer has moved its vertices of batch b-1
-        if (b >= 1 && !wait_rounds(ctrB, a.NR, b, a.err, a.ticks, &sflag)) return;
-        const int myn = bc > r ? (bc - r + a.NR - 1) / a.NR : 0;
-        for (int k = wave; k < myn; k += WG / 64) {  // corrections, one wave per vertex
-            const int i = b0 + r + k * a.NR;
-            const double2 pi = a.pos[i];
-            double cx = 0.0, cy = 0.0;
-            const int kend = a.erp[i + 1];
-            for (int e = a.erp[i] + lane; e < kend; e += 64) {
-                const double2 pj = a.pos[a.eci[e]];
-                correct_pair<true>(a.law, a.ew ? a.W * (double)a.ew[e] : a.W, pj.x - pi.x, pj.y - pi.y, cx, cy);
-            }
-            cx = wave_sum(cx);
-            cy = wave_sum(cy);
-            if (lane == 0) corr[k] = make_double2(cx, cy);
-        }
-        signal_done(ctrC, r);
-        // all partials of batch b, and every correction of batch b (no vertex of the batch
-        // moves before all of them have read the batch's start positions)
-        if (!wait_rounds(ctrA, a.NW, b + 1, a.err, a.ticks, &sflag)) return;
-        if (!wait_rounds(ctrC, a.NR, b + 1, a.err, a.ticks, &sflag)) return;
-        for (int k = 0; k < myn; ++k) {
-            const int s = r + k * a.NR;
-            const double2 *row = a.part + ((size_t)buf * a.Bmax + s) * a.NW;
-            double sx = 0.0, sy = 0.0;
-            for (int p = tid; p < a.NW; p += WG) {
-                const double2 v = row[p];
-                sx += v.x;
-                sy += v.y;
-            }
-            sx = wave_sum(sx);
-            sy = wave_sum(sy);
-            if (lane == 0) red[wave] = make_double2(sx, sy);
-            __syncthreads();
-            if (tid == 0) {
-                const double fx = ((red[0].x + red[1].x) + (red[2].x + red[3].x)) + corr[k].x;
-                const double fy = ((red[0].y + red[1].y) + (red[2].y + red[3].y)) + corr[k].y;
-                move_one(a.pos, b0 + s, fx, fy, a.step, a.eterm);
-            }
-            __syncthreads();
-        }
-        signal_done(ctrB, r);
-    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -407,41 +221,6 @@ __global__ __launch_bounds__(WG) void k_move64(const double2 *__restrict__ part,
     move_one(pos, b0 + slot, sx, sy, step, eterm);
 }
 
-// Geometry of the persistent launch for a graph (0 workers = not possible).
-struct IterPlan {
-    int NW = 0, NR = 0, chunk = 0;
-    size_t lds = 0;
-};
-
-IterPlan plan_iter(int n, int bmax) {
-    IterPlan pl;
-    int dev = 0, cus = 0, coop = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0 ||
-        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop)
-        return pl;
-    pl.NR = std::min(256, std::max(32, (bmax + 7) / 8));
-    if ((bmax + pl.NR - 1) / pl.NR > MAX_SLOTS) return IterPlan();
-    for (int per_cu = 4; per_cu >= 1; --per_cu) {
-        int nw = per_cu * cus - pl.NR;
-        if (nw < 1) continue;
-        nw = std::min(nw, std::max(1, (n + 15) / 16));  // at least 16 vertices per chunk
-        const int chunk = (n + nw - 1) / nw;
-        const size_t lds = sizeof(double2) * (size_t)chunk;
-        if (lds > 64 * 1024) continue;
-        int fit = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_iter32, WG, lds) != hipSuccess) continue;
-        const int workers = (n + chunk - 1) / chunk;
-        if ((long)fit * cus >= (long)(workers + pl.NR)) {
-            pl.NW = workers;
-            pl.chunk = chunk;
-            pl.lds = lds;
-            return pl;
-        }
-    }
-    return IterPlan();
-}
-
 }  // namespace
 
 // ---- energy reductions (shared with the Barnes-Hut path) -------------------------------
@@ -509,7 +288,6 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     const float *ew = g->has_w ? g->eff_w.as<float>() : nullptr;
 
     // launch geometry
-    IterPlan pl;
     int JS = 1, G = 1, chunk = n, P = 1;
     size_t part_elems = 0;
     if (fp64) {
@@ -517,7 +295,6 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
         P = G;
         part_elems = (size_t)bmax * P;
     } else {
-        // per-step geometry (also the fallback when the persistent launch is refused)
         // 64 batch slots x 4 j-subgroups per workgroup: at least ~1024 workgroups (4 per CU),
         // and j chunks of at most ~CHUNK_MAX positions, whose scalar loads the rows of a chunk
         // then mostly share in the scalar cache
@@ -532,43 +309,13 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
         G = (n + chunk - 1) / chunk;
         P = G + g->corr_max_pieces;
         part_elems = (size_t)bmax * P;
-        if (getenv("BL_PERSISTENT") && p->precision == BL_PREC_MIXED) pl = plan_iter(n, bmax);
-        if (pl.NW > 0) part_elems = std::max(part_elems, 2 * (size_t)bmax * pl.NW);
     }
     if (g->part.ensure(sizeof(double2) * part_elems)) return BL_E_NOMEM;
     if (g->eterm.ensure(sizeof(double) * (size_t)n)) return BL_E_NOMEM;
     if (g->energy.ensure(sizeof(double) * (4 + ENERGY_PARTS))) return BL_E_NOMEM;
-    const size_t ctr_bytes = sizeof(unsigned) * 3 * NSTRIPE * CSTRIDE + 256;
-    if (pl.NW > 0 && g->scratch.ensure(std::max(ctr_bytes, g->scratch.bytes))) return BL_E_NOMEM;
     double2 *part = g->part.as<double2>();
     double *eterm = g->eterm.as<double>();
     double *energy = g->energy.as<double>();
-
-    IterArgs ia;
-    if (pl.NW > 0) {
-        int dev = 0, khz = 0;
-        BL_HIP(hipGetDevice(&dev));
-        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0)
-            khz = 100000;
-        ia.pos = pos;
-        ia.n = n;
-        ia.B = B;
-        ia.nb = nb;
-        ia.NW = pl.NW;
-        ia.NR = pl.NR;
-        ia.chunk = pl.chunk;
-        ia.Bmax = bmax;
-        ia.part = part;
-        ia.erp = erp;
-        ia.eci = eci;
-        ia.ew = ew;
-        ia.W = p->weight;
-        ia.law = law;
-        ia.eterm = eterm;
-        ia.ctr = g->scratch.as<unsigned>();
-        ia.err = reinterpret_cast<int *>(ia.ctr + 3 * NSTRIPE * CSTRIDE);
-        ia.ticks = (long long)khz * 1000LL * 2;  // 2 s per wait
-    }
 
     Prof prof;
     prof.on = p->profile != 0;
@@ -583,25 +330,7 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
             break;
         }
         E0 = E;
-        bool done = false;
-        if (pl.NW > 0) {
-            BL_HIP(hipMemsetAsync(ia.ctr, 0, ctr_bytes, s));
-            ia.step = STEP;
-            void *args[] = {&ia};
-            prof.begin(0);
-            const hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_iter32),
-                                                            dim3(pl.NW + pl.NR), dim3(WG), args, (unsigned)pl.lds, s);
-            prof.end();
-            if (e == hipSuccess) {
-                done = true;
-            } else {  // refused (nothing ran): per-step launches from here on
-                (void)hipGetLastError();
-                fprintf(stderr, "batchlayout: persistent launch refused (%s); using per-step launches\n",
-                        hipGetErrorString(e));
-                pl.NW = 0;
-            }
-        }
-        if (!done) {
+        {
             for (int b = 0; b < nb; ++b) {
                 const int b0 = b * B;
                 const int bc = (n - b0) < B ? (n - b0) : B;
@@ -649,13 +378,7 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     if (!converged && loop > 0) {
         BL_HIP(hipMemcpyAsync(&E, energy, sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    int err = 0;
-    if (pl.NW > 0 && loop > 0) BL_HIP(hipMemcpyAsync(&err, ia.err, sizeof(int), hipMemcpyDeviceToHost, s));
     BL_HIP(hipStreamSynchronize(s));
-    if (err) {
-        fprintf(stderr, "batchlayout: a wait of the persistent minibatch kernel passed its deadline\n");
-        return BL_E_DEVICE;
-    }
     prof.report(st);
     if (st) {
         st->energy = loop > 0 ? E : (law == LAW_FR ? 0.0 : 1.79769313486231570815e+308);

@@ -23,6 +23,9 @@ import numpy as np
 
 from . import api, graphs
 
+# the library's kernel classes (bl_stats.kernel_ms)
+CLASSES = ("pairs_or_traversal", "move_or_walk", "energy", "tree", "cell_summaries", "walk_prepass")
+
 
 def _run(tag, rp, ci, w, iters, profile, **kw):
     n = len(rp) - 1
@@ -33,8 +36,8 @@ def _run(tag, rp, ci, w, iters, profile, **kw):
     rec = dict(workload=tag, n=n, nnz=int(len(ci)), iters=iters, seconds=st.seconds,
                vertex_updates_per_s=n * iters / st.seconds, energy=st.energy, params=kw)
     if profile:
-        rec["kernel_ms_per_iter"] = dict(zip(("pairs_or_repulsion", "move_or_walk", "energy", "tree"),
-                                             [m / iters for m in st.kernel_ms]))
+        rec["kernel_ms_per_iter"] = dict(zip(CLASSES, [m / iters for m in st.kernel_ms]))
+        rec["walk"] = st.walk
     return rec
 
 
@@ -78,8 +81,8 @@ def main(argv=None):
                 rec = dict(workload=a.name, mode=mode, n=n, nnz=int(len(ci)), iters=a.iters, seconds=st.seconds,
                            vertex_updates_per_s=n * a.iters / st.seconds, energy=st.energy)
                 if a.profile:
-                    rec["kernel_ms_per_iter"] = dict(zip(("repulsion", "walk", "energy", "tree"),
-                                                         [m / a.iters for m in st.kernel_ms]))
+                    rec["kernel_ms_per_iter"] = dict(zip(CLASSES, [m / a.iters for m in st.kernel_ms]))
+                    rec["far_cells_per_iter"] = st.far_cells
                 out.append(rec)
                 print(json.dumps(rec), flush=True)
         out = []

@@ -51,7 +51,11 @@ extern "C" {
                                 mass-weighted monopole m (c - v)/|c - v|^2 */
 #define BL_ALGO_FAR_EXACT 41 /* the exact sum over all vertices (O(n^2) per iteration) */
 #define BL_ALGO_FAR_LEARNED 42 /* monopoles corrected by the learned far-field model set with
-                                  bl_graph_set_far_model (BL_E_ARG without one) */
+                                  bl_graph_set_far_model (BL_E_ARG without one).  Judged at run
+                                  time: every 10 iterations from the first, the exact repulsion of a
+                                  sample of 256 vertices decides whether the model (median relative
+                                  force error at most the analytic term's) or the analytic
+                                  second-order correction (algo 43) serves the next iterations */
 #define BL_ALGO_FAR_QUAD 43  /* monopoles corrected by the analytic second-order multipole term */
 
 /* ---- precision of the pair arithmetic -------------------------------------------- */
@@ -88,11 +92,34 @@ typedef struct {
     double seconds;       /* wall time of the layout loop (the reference's end - start) */
     double step_final;    /* STEP after the last iteration */
     /* with bl_params.profile: GPU time (ms) and launch count per kernel class:
-       [0] pair forces / tree traversal, [1] reduce+move / attraction+move,
-       [2] energy reduction, [3] tree build (bounding box, keys, sort, nodes, centroids) */
-    double kernel_ms[4];
-    int32_t kernel_launches[4];
+       [0] pair forces / tree traversal, [1] reduce+move / the batch walk,
+       [2] energy reduction, [3] tree build (bounding box, keys, sort, nodes, centroids),
+       [4] far-field cell summaries (algos 40, 42, 43), [5] the batch walk's pre-pass
+       (attraction of the neighbours that have not moved yet), [6] the learned far field's judge,
+       [7] unused */
+    double kernel_ms[8];
+    int32_t kernel_launches[8];
+    /* Barnes-Hut family (algos 4, 6, 40-43): the batch walk of the last iteration, one of BL_WALK_*
+       (0 for the all-pairs algorithms) */
+    int32_t walk;
+    /* algos 40, 42, 43: cells the far-field traversal accepted in the last iteration (with the
+       learned model, the number of model evaluations); 0 otherwise */
+    int64_t far_cells;
+    /* algo 42: iterations whose traversal used the learned model (the others fell back to the
+       analytic second-order correction), and the median relative force errors of the learned and
+       of the analytic field against the exact sum on the judge's last sample */
+    int32_t far_model_iters;
+    float far_judge_err[2];
 } bl_stats;
+
+/* The batch walks of the Barnes-Hut family (bl_stats.walk); the library picks one per graph and
+ * batch size (tests may force one with the environment variable BL_WALK=grid|single|pf|row) */
+#define BL_WALK_NONE 0
+#define BL_WALK_ROW_ORDER 1  /* FP64 mode: the reference's row order, one workgroup */
+#define BL_WALK_PF 2         /* one workgroup, a batch's CSR entries split evenly over its threads */
+#define BL_WALK_SPLIT 3      /* one workgroup, batches larger than the prefetching walk takes */
+#define BL_WALK_ROW 4        /* one workgroup, one thread per vertex, short rows */
+#define BL_WALK_GRID 5       /* several co-resident workgroups, one grid barrier per batch */
 
 /* Fills *p with the reference CLI defaults. */
 int bl_params_default(bl_params *p);
