@@ -25,7 +25,7 @@ FAR_MODEL_PARAMS, FAR_FEATURES = 9026, 8
 PREC_MIXED, PREC_FP64, PREC_FP32 = 0, 1, 2
 BATCHED_MAX_N = 8192
 # the Barnes-Hut family's batch walks (Stats.walk)
-WALK_NONE, WALK_ROW_ORDER, WALK_PF, WALK_SPLIT, WALK_ROW, WALK_GRID = 0, 1, 2, 3, 4, 5
+WALK_NONE, WALK_ROW_ORDER, WALK_PF, WALK_SPLIT, WALK_ROW, WALK_GRID, WALK_LAG = 0, 1, 2, 3, 4, 5, 6
 
 
 class BLError(RuntimeError):
@@ -44,7 +44,7 @@ class bl_stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("iters_done", C.c_int32), ("status", C.c_int32),
                 ("seconds", C.c_double), ("step_final", C.c_double), ("kernel_ms", C.c_double * 8),
                 ("kernel_launches", C.c_int32 * 8), ("walk", C.c_int32), ("far_cells", C.c_int64),
-                ("far_model_iters", C.c_int32), ("far_judge_err", C.c_float * 2)]
+                ("far_model_iters", C.c_int32), ("far_judge_err", C.c_float * 2), ("walk_helped", C.c_int32)]
 
 
 _lib = None
@@ -120,11 +120,13 @@ class Stats:
     far_cells: int = 0
     far_model_iters: int = 0
     far_judge_err: tuple = (0.0, 0.0)
+    walk_helped: int = 0
 
 
 def _stats(st: bl_stats) -> Stats:
     return Stats(st.energy, st.iters_done, st.seconds, st.step_final, tuple(st.kernel_ms),
-                 tuple(st.kernel_launches), st.walk, st.far_cells, st.far_model_iters, tuple(st.far_judge_err))
+                 tuple(st.kernel_launches), st.walk, st.far_cells, st.far_model_iters, tuple(st.far_judge_err),
+                 st.walk_helped)
 
 
 class Graph:

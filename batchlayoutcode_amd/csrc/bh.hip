@@ -2023,6 +2023,19 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         else if (!strcmp(wenv, "row")) use_grid = false;
     }
     if (use_grid) use_row = false;
+    // the lagged walk (helper workgroups) where the row walk fits and the graph has enough batches;
+    // BL_LAG_D / BL_LAG_H / BL_LAG_RK set its lag, helper count and publishing period
+    int lag_d = 16, lag_h = 32, lag_rk = 4;
+    if (const char *v = getenv("BL_LAG_D")) lag_d = atoi(v);
+    if (const char *v = getenv("BL_LAG_H")) lag_h = std::max(0, atoi(v));
+    if (const char *v = getenv("BL_LAG_RK")) lag_rk = std::max(1, atoi(v));
+    bool use_lag = use_row && lag_walk_fits(g, p->batch, lag_d);
+    if (const char *wenv = getenv("BL_WALK"))
+        if (!strcmp(wenv, "row")) use_lag = false;
+    if (use_lag) {
+        rc = lag_walk_prepare(g->has_w);
+        if (rc) return rc;
+    }
     if (use_row) {
         const size_t lb = wrow_lds(g->has_w);
         BL_HIP(hipFuncSetAttribute(g->has_w ? reinterpret_cast<const void *>(k_walk_row<true>)
@@ -2087,6 +2100,10 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->bh.misc.as<int>());
         } else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK) {
             walk = BL_WALK_GRID;
+        } else if (use_lag) {
+            walk = BL_WALK_LAG;
+            rc = launch_walk_lag(g, pos, p, STEP, lag_h, lag_rk, s);
+            if (rc) return rc;
         } else if (use_row) {
             walk = BL_WALK_ROW;
             hipLaunchKernelGGL(g->has_w ? k_walk_row<true> : k_walk_row<false>, dim3(1), dim3(2 * WROW_T),
@@ -2118,6 +2135,8 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     unsigned long long cells = 0;
     if (far_mode && far_mode != FAR_EXACT && iters > 0)
         BL_HIP(hipMemcpyAsync(&cells, ncells, sizeof(cells), hipMemcpyDeviceToHost, s));
+    int helped = 0;
+    if (walk == BL_WALK_LAG) BL_HIP(hipMemcpyAsync(&helped, lag_walk_helped(g), sizeof(int), hipMemcpyDeviceToHost, s));
     int model_iters = far_mode == FAR_LEARNED ? iters : 0;
     float judge_err[2] = {0.0f, 0.0f};
     if (judged && iters > 0) {
@@ -2138,6 +2157,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     if (st) {
         st->far_cells = (int64_t)cells;
         st->far_model_iters = model_iters;
+        st->walk_helped = helped;
         st->far_judge_err[0] = judge_err[0];
         st->far_judge_err[1] = judge_err[1];
         st->energy = E;

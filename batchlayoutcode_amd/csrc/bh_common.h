@@ -29,6 +29,16 @@ int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 // w.rep (by vertex id).  FAR_EXACT sums over all points; the other modes read the tree of
 // d_pos already built in w by bh_build_tree.  FAR_LEARNED needs model (device, MLP_NPARAM
 // floats).
+// The lagged batch walk (walk_lag.hip): lag_walk_fits builds the settled / recent split for
+// batch size B and lag D (false: the walk does not take this graph and batch size);
+// launch_walk_lag runs one iteration's walk with `helpers` helper workgroups, publishing the
+// walker's progress every rk batches (after k_late_attract); lag_walk_prepare sets the kernels'
+// LDS limit.
+bool lag_walk_fits(bl_graph *g, int B, int D);
+int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, int rk, hipStream_t s);
+int lag_walk_prepare(bool has_w);
+int *lag_walk_helped(bl_graph *g);
+
 // The other modes need the cell summaries of that tree first (far_cells_of).  ncells
 // (device, nullable) is incremented by the number of cells the traversal accepts.
 enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3, FAR_QUAD = 4 };

@@ -110,6 +110,9 @@ typedef struct {
        of the analytic field against the exact sum on the judge's last sample */
     int32_t far_model_iters;
     float far_judge_err[2];
+    /* BL_WALK_LAG: batches of the last iteration whose sums over long-moved neighbours came from the
+       helper workgroups (the walker formed the others itself) */
+    int32_t walk_helped;
 } bl_stats;
 
 /* The batch walks of the Barnes-Hut family (bl_stats.walk); the library picks one per graph and
@@ -120,6 +123,8 @@ typedef struct {
 #define BL_WALK_SPLIT 3      /* one workgroup, batches larger than the prefetching walk takes */
 #define BL_WALK_ROW 4        /* one workgroup, one thread per vertex, short rows */
 #define BL_WALK_GRID 5       /* several co-resident workgroups, one grid barrier per batch */
+#define BL_WALK_LAG 6        /* one walking workgroup (one thread per vertex, short rows) and helper
+                                workgroups that sum the neighbours moved at least 16 batches earlier */
 
 /* Fills *p with the reference CLI defaults. */
 int bl_params_default(bl_params *p);

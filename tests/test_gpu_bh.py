@@ -258,3 +258,51 @@ def test_row_walk_matches_entry_split_walk(bl, monkeypatch, kind, weighted, batc
     d = diameter(out["pf"])
     assert np.abs(out["single"] - out["pf"]).max() <= 1e-10 * d
     assert abs(out["single_E"] - out["pf_E"]) <= 1e-10 * out["pf_E"]
+
+
+@pytest.mark.parametrize("kind,weighted,batch,algo", [("rgg", False, 256, 4), ("rgg", True, 256, 4), ("rgg", False, 100, 4),
+                                                      ("grid", False, 256, 4), ("delaunay", False, 256, 40),
+                                                      ("delaunay", True, 200, 4)])
+def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, weighted, batch, algo):
+    """The lagged walk (helper workgroups sum the neighbours moved at least D batches earlier; the
+    walker sums the rest and forms a batch's sum itself when its helper is late) against the
+    one-workgroup row walk (partial sums in another order: 1e-10 of the diameter after three
+    iterations), and bit for bit against itself with no helpers at all and with other lags and
+    publishing periods: whichever workgroup formed a batch's sum, it is the same."""
+    if kind == "grid":
+        rp, ci = graphs.csr_from_edges(*graphs.grid(128, 128))
+    elif kind == "delaunay":
+        rp, ci = graphs.csr_from_edges(*graphs.delaunay_holes(40_000, seed=3))
+    else:
+        rp, ci = graphs.csr_from_edges(*graphs.rgg(60_000, seed=5))
+    n = len(rp) - 1
+    w = np.random.default_rng(7).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
+    x0 = np.random.default_rng(8).random((n, 2))
+    print(f"[case] lag walk {kind} weighted {weighted} batch {batch} algo {algo} n {n}", flush=True)
+    out, helped = {}, {}
+    # a short lag has the helpers read positions the walker has just published
+    for mode, env in (("row", dict(BL_WALK="row")), ("lag", {}), ("lag_h0", dict(BL_LAG_H="0")),
+                      ("lag_rk1", dict(BL_LAG_RK="1")), ("lag_d8", dict(BL_LAG_D="8", BL_LAG_RK="1")),
+                      ("lag_d8_h0", dict(BL_LAG_D="8", BL_LAG_H="0"))):
+        for k in ("BL_WALK", "BL_LAG_H", "BL_LAG_RK", "BL_LAG_D"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        with bl.Graph(rp, ci, w) as g:
+            out[mode], st = g.run(x0, algo=algo, iters=3, batch=batch, bht=1.2)
+            out[mode + "_E"] = st.energy
+            helped[mode] = st.walk_helped
+            assert st.walk == (bl.WALK_ROW if mode == "row" else bl.WALK_LAG), (mode, st.walk)
+    nb = (n + batch - 1) // batch
+    print(f"batches {nb}, formed by the helpers (last iteration): {helped}")
+    d = diameter(out["row"])
+    # partial sums in another order: rounding differences, grown over three chaotic iterations
+    assert np.abs(out["lag"] - out["row"]).max() <= 1e-9 * d
+    assert abs(out["lag_E"] - out["row_E"]) <= 1e-9 * out["row_E"]
+    assert helped["lag_h0"] == 0 and helped["lag_d8_h0"] == 0
+    if kind != "grid":  # on the grid every early neighbour is recent: the helpers' sums are trivial
+        assert helped["lag"] > nb // 2 and helped["lag_d8"] > 0
+    assert np.array_equal(out["lag"], out["lag_h0"]) and out["lag_E"] == out["lag_h0_E"]
+    assert np.array_equal(out["lag"], out["lag_rk1"]) and out["lag_E"] == out["lag_rk1_E"]
+    assert np.array_equal(out["lag_d8"], out["lag_d8_h0"]) and out["lag_d8_E"] == out["lag_d8_h0_E"]
+    assert np.abs(out["lag_d8"] - out["row"]).max() <= 1e-9 * d

@@ -15,9 +15,10 @@ seed 3, LCC; physically consistent Barnes-Hut, algos 40-43):
   * reduced (40,000 points): algo 41 and algo 40 (theta 1.0) iterations against the oracle's
     layout loop (FP64 sums in another order: 1e-8 of the diameter); after 10 iterations, the
     monopole, quadrupole and learned repulsion at the layout's positions against the oracle's exact
-    FP64 sum on a sample of vertices (the quadrupole beats the monopole); the learned field's judge
-    against the oracle on its own sample (its medians and its verdict), and one more iteration of
-    the judged algo 42 within 1% of algo 41's energy;
+    FP64 sum on a sample of vertices (the quadrupole and the model at most half the monopole's
+    median error); the learned field's judge against the oracle on its own sample (its medians and
+    its verdict), and one more iteration of the judged algo 42 closer to algo 41's energy than the
+    monopole's by half;
   * full size (1,662,995 vertices): the same after 10 iterations, on a seeded 10,000-vertex sample
     against the exact sum (the GPU's FP64 exact sum, itself checked against the oracle on 64 of
     the vertices).
@@ -159,7 +160,7 @@ def test_delaunay_reduced_far_layouts_vs_oracle(bl, oracle, far_model):
     for algo, theta in ((41, 0.0), (40, 1.0)):
         ref, e_ref = oracle.far_layout(rp, ci, x0, 1, 256, theta)
         xy, st = out[algo]
-        assert st.walk == bl.WALK_ROW
+        assert st.walk == bl.WALK_LAG
         assert np.abs(xy - ref).max() <= 1e-8 * diameter(ref), algo
         assert abs(st.energy - e_ref) <= 1e-8 * e_ref, algo
     # the far-field laws at the layout's positions against the exact FP64 sum
@@ -173,9 +174,11 @@ def test_delaunay_reduced_far_layouts_vs_oracle(bl, oracle, far_model):
     de = {a: (out10[a][1].energy - e41) / e41 for a in (40, 42, 43)}
     print(f"delaunay40k after 10 iterations, theta 1.0: median force error {med}; energy of the next iteration "
           f"vs exact {de}")
-    assert med["mono"] < 0.02 and med["quad"] < med["mono"]
+    # a layout's clustered positions: the monopole is several percent off at theta 1.0, the
+    # second-order term and the model remove most of that
+    assert med["mono"] < 0.15 and med["quad"] < 0.5 * med["mono"] and med["learned"] < 0.5 * med["mono"]
     _check_judge(bl, oracle, x10, out10[42][1], 1.0, far_model)
-    assert abs(de[42]) <= 0.01
+    assert abs(de[42]) < 0.5 * abs(de[40])
 
 
 @pytest.mark.timeout(900)
@@ -187,7 +190,7 @@ def test_delaunay2m_far_field_vs_exact(bl, oracle, far_model):
     with bl.Graph(rp, ci) as g:
         g.set_far_model(far_model)
         x10, st10 = g.run(x0, algo=40, iters=10, batch=256, bht=1.0)
-        assert st10.walk == bl.WALK_ROW
+        assert st10.walk == bl.WALK_LAG
         es = {a: g.run(x10, algo=a, iters=1, batch=256, bht=1.0, step0=0.999 ** 10)[1] for a in (41, 40, 42, 43)}
     sample = np.random.default_rng(3).choice(n, 10_000, replace=False)
     q = x10[sample]
@@ -203,6 +206,6 @@ def test_delaunay2m_far_field_vs_exact(bl, oracle, far_model):
     de = {a: (es[a].energy - es[41].energy) / es[41].energy for a in (40, 42, 43)}
     print(f"delaunay2m theta 1.0 relative force error: median {med}, p99 {p99}; energy of one iteration vs "
           f"exact {de}", flush=True)
-    assert med["mono"] < 0.02 and med["quad"] < med["mono"]
+    assert med["mono"] < 0.15 and med["quad"] < 0.5 * med["mono"] and med["learned"] < 0.5 * med["mono"]
     _check_judge(bl, oracle, x10, es[42], 1.0, far_model)
-    assert abs(de[42]) <= 0.01
+    assert abs(de[42]) < 0.5 * abs(de[40])
