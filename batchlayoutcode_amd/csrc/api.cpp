@@ -176,8 +176,8 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
     DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
                      &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model, &g->cpre, &g->cpv,
                      &g->early_rp, &g->early_ci, &g->early_w, &g->late_rp, &g->late_ci, &g->late_w,
-                     &g->grid_rows, &g->lag_erp, &g->lag_esp, &g->lag_eci, &g->lag_ew, &g->lag_rrp, &g->lag_rci,
-                     &g->lag_rw, &g->lag_pre, &g->lag_ready};
+                     &g->grid_rows, &g->lag_erp, &g->lag_esp, &g->lag_eci, &g->lag_ew, &g->lag_rofs, &g->lag_rw,
+                     &g->lag_pre, &g->lag_ready};
     for (DevBuf *b : all) b->release();
     g->bh.release();
     if (g->stream) (void)hipStreamDestroy(g->stream);

@@ -2024,11 +2024,10 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     }
     if (use_grid) use_row = false;
     // the lagged walk (helper workgroups) where the row walk fits and the graph has enough batches;
-    // BL_LAG_D / BL_LAG_H / BL_LAG_RK set its lag, helper count and publishing period
-    int lag_d = 16, lag_h = 32, lag_rk = 4;
+    // BL_LAG_D / BL_LAG_H set its lag and helper count
+    int lag_d = 20, lag_h = 32;
     if (const char *v = getenv("BL_LAG_D")) lag_d = atoi(v);
     if (const char *v = getenv("BL_LAG_H")) lag_h = std::max(0, atoi(v));
-    if (const char *v = getenv("BL_LAG_RK")) lag_rk = std::max(1, atoi(v));
     bool use_lag = use_row && lag_walk_fits(g, p->batch, lag_d);
     if (const char *wenv = getenv("BL_WALK"))
         if (!strcmp(wenv, "row")) use_lag = false;
@@ -2102,7 +2101,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
             walk = BL_WALK_GRID;
         } else if (use_lag) {
             walk = BL_WALK_LAG;
-            rc = launch_walk_lag(g, pos, p, STEP, lag_h, lag_rk, s);
+            rc = launch_walk_lag(g, pos, p, STEP, lag_h, s);
             if (rc) return rc;
         } else if (use_row) {
             walk = BL_WALK_ROW;

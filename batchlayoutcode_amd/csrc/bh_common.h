@@ -31,11 +31,10 @@ int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 // floats).
 // The lagged batch walk (walk_lag.hip): lag_walk_fits builds the settled / recent split for
 // batch size B and lag D (false: the walk does not take this graph and batch size);
-// launch_walk_lag runs one iteration's walk with `helpers` helper workgroups, publishing the
-// walker's progress every rk batches (after k_late_attract); lag_walk_prepare sets the kernels'
-// LDS limit.
+// launch_walk_lag runs one iteration's walk with `helpers` helper workgroups (after
+// k_late_attract); lag_walk_prepare sets the kernels' LDS limit.
 bool lag_walk_fits(bl_graph *g, int B, int D);
-int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, int rk, hipStream_t s);
+int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, hipStream_t s);
 int lag_walk_prepare(bool has_w);
 int *lag_walk_helped(bl_graph *g);
 

@@ -76,11 +76,12 @@ struct bl_graph {
     int32_t grid_rows_batch = 0;
     bl::DevBuf grid_rows;
     // the lagged walk (walk_lag.hip) at batch size lag_batch and lag lag_d: each vertex's early
-    // entries split into settled (at least lag_d batches back) and recent ones, the recent list
-    // alone, the helpers' per-vertex sums and per-batch flags (launch generations)
+    // entries split into settled (at least lag_d batches back) and recent ones, the recent ones'
+    // ring offsets and weights (4 per vertex), the helpers' per-vertex sums and per-batch flags
+    // (launch generations)
     int32_t lag_batch = 0, lag_d = 0, lag_gen = 0;
     bool lag_ok = false;
-    bl::DevBuf lag_erp, lag_esp, lag_eci, lag_ew, lag_rrp, lag_rci, lag_rw, lag_pre, lag_ready;
+    bl::DevBuf lag_erp, lag_esp, lag_eci, lag_ew, lag_rofs, lag_rw, lag_pre, lag_ready;
     // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),
     // as a CSR of strictly increasing column ids per row (+ their weights)
     bl::DevBuf eff_rowptr, eff_colids, eff_w;

@@ -2,33 +2,38 @@
 // attraction of every CSR entry with live positions minus the repulsion, then the sequential
 // move), for batches of at most LAG_T vertices whose rows are short.
 //
-// The reference spreads each batch over its threads; a batch's attraction reads the positions its
-// neighbours have at that moment, so batch b depends on every earlier batch that holds one of its
-// neighbours.  On graphs with unordered ids those neighbours are spread over all earlier batches,
-// and only a few lie in the last D-1 batches.  The walk therefore splits each vertex's "early"
-// neighbours (those in earlier batches, which have moved) into
-//   settled  in batches <= b - D: final once batch b - D has moved, and
-//   recent   in batches b - D + 1 .. b - 1,
-// (the "late" ones, in batch b or later, still at their iteration-start positions, are summed
-// beforehand by k_late_attract into the repulsion rep').
+// A batch's attraction reads the positions its neighbours have at that moment, so batch b
+// depends on every earlier batch that holds one of its neighbours.  On graphs with unordered ids
+// those neighbours are spread over all earlier batches and few lie in the last D - 1.  Each
+// vertex's "early" neighbours (in earlier batches, moved this iteration) are therefore split into
+//   settled  in batches <= b - D, final once batch b - D has moved, and
+//   recent   in batches b - D + 1 .. b - 1 (at most LAG_RS per vertex);
+// the "late" ones (batch b or later, still at their iteration-start positions) are summed
+// beforehand by k_late_attract into the repulsion: rep'_i = rep_i - sum_late attraction.
 //
-//   workgroup 0, the walker: the one-workgroup row walk (LAG_T computing threads, one per batch
-//     vertex, and LAG_T loading threads that stage the next batch in LDS), with
-//        f_i = pre_i + sum_{recent} attraction,   pre_i = sum_{settled} attraction - rep'_i;
-//   workgroups 1..H, the helpers: helper h takes batches h-1, h-1+H, ...; for batch b it waits
-//     until the walker has published batches [0, b - D + 1) (batches it needs are final in
-//     memory), computes pre_i of the batch's vertices into memory and flags the batch with the
-//     launch's generation number.
-// The walker never waits for a helper: three batches ahead it reads the batch's flag; a flagged
-// batch is staged with its pre_i and recent entries only, an unflagged one with all its early
-// entries, and the walker forms pre_i itself with the same FP64 operations in the same (CSR)
-// order, so the layout is the same whichever workgroup formed it.  Helpers stop at a deadline or
-// when the walker has finished, so the launch ends whatever the scheduling.
+//   f_i = pre_i + sum_{recent} attraction,     pre_i = sum_{settled} attraction - rep'_i
 //
-// Publishing: after the stores of batch c have completed, one loading thread of the walker
-// releases done = c + 1 every LAG_RK batches (an agent-scope release writes the XCD's L2 back);
-// a helper acquires it before gathering.  A helper's flag is released after its pre_i stores;
-// the walker's loading waves acquire before loading a flagged batch's pre_i.
+// Workgroups 1..H, the helpers: helper h takes batches h-1, h-1+H, ...; it loads what does not
+// change (its vertices' positions, rep', settled column ids) at once, waits until the walker has
+// published batches [0, b - D + 1), gathers the settled positions, writes pre_i of the batch to
+// memory and flags the batch with the launch's generation number.
+//
+// Workgroup 0, the walker, never waits for a helper:
+//   computing waves (LAG_T threads, thread t owns vertex b0 + t of every batch) read only LDS: a
+//     staged batch (own position, pre_i or rep'_i, the ring slots of the recent neighbours) and a
+//     ring of the last D batches' moved positions, where every recent neighbour is found.  A
+//     batch without a helper's flag forms pre_i itself from memory, with the same FP64 operations
+//     in the same (CSR) order as a helper, so the layout is the same whichever formed it.  Moved
+//     positions and energy terms go straight to memory;
+//   loading waves stage batch b + 3 during batch b with direct global-to-LDS loads (4 slots), and
+//     wait only for the loads of batch b + 1, so three batches of loads are in flight;
+//   a control wave reads the helpers' flags two batches at a time, LAG_LEAD batches ahead,
+//     acquires, and records for each batch, before the batch is staged, whether its pre_i is used;
+//     every second batch it publishes the walker's progress to the helpers (batches whose stores
+//     completed: the computing waves wait for the stores of three batches back).
+// Barriers are raw s_barrier with explicit LDS waits (a workgroup barrier would also wait for the
+// loads in flight).  Helpers stop at a deadline or when the walker has finished, so the launch
+// ends whatever the scheduling.
 #include <math.h>
 #include <stdlib.h>
 
@@ -40,43 +45,56 @@
 namespace bl {
 namespace {
 
-constexpr int LAG_T = 256;                 // batch vertices; walker computing / loading threads
-constexpr int LAG_CAP = 2560;              // staged CSR entries per batch
-constexpr int LAG_S = LAG_CAP / LAG_T;     // staged entries per loading thread
-constexpr int LAG_MAXDEG = 64;             // longest row for which this walk is chosen
+typedef __attribute__((address_space(3))) void lds_void;
+
+constexpr int LAG_T = 256;              // batch vertices; the walker's computing and loading threads
+constexpr int LAG_WG = 2 * LAG_T + 64;  // + one control wave
+constexpr int LAG_K = 4;                // stage slots: batch b + 3 is loaded during batch b
+constexpr int LAG_DMAX = 20;            // largest lag (ring slots of moved positions)
+constexpr int LAG_RS = 4;               // recent neighbours per vertex (ring offsets in an int4)
+constexpr int LAG_LEAD = 6;             // a batch's flag is read LAG_LEAD or LAG_LEAD + 1 batches ahead
+constexpr int LAG_DEC = 16;             // decision ring (batch % LAG_DEC)
+constexpr int LAG_MAXDEG = 64;          // longest row for which this walk is chosen
+constexpr int LAG_HS = 8;               // settled entries loaded together (helpers: before the wait)
 
 struct LagArgs {
-    const int *erp, *esp, *eci;  // early entries of vertex i: [erp[i], esp[i]) settled, [esp[i], erp[i+1]) recent
+    const int *erp, *esp, *eci;  // early entries of vertex i: [erp[i], esp[i]) settled, then recent
     const float *ew;
-    const int *rrp, *rci;        // recent entries only
-    const float *rw;
+    const int4 *rofs;            // ring offsets of vertex i's recent neighbours (-1: none), CSR order
+    const float4 *rw;            // their weights
     double W;
     double2 *pos;
     const double2 *rep;          // repulsion minus the late attraction (k_late_attract)
     double2 *pre;                // pre_i written by the helpers
     int *ready;                  // ready[b] == gen: pre of batch b is in memory
-    int *done;                   // batches [0, *done) of this launch are final in memory
+    int *done;                   // batches [0, *done) of this launch are final in memory; done[1]: helped
     int gen;
-    int n, B, nb, D, RK;
+    int n, B, nb, D;
     double step;
     double *eterm;
     const int *status;
     long long ticks;             // a helper's longest wait
 };
 
-constexpr size_t lag_lds(bool has_w) {
-    return sizeof(double2) * (3 * LAG_T + 2 * LAG_CAP + 4 * LAG_T) + sizeof(double) * 2 * LAG_T +
-           sizeof(int) * (2 * LAG_CAP + 6 * LAG_T + 4) + (has_w ? sizeof(float) * 2 * LAG_CAP : 0);
+size_t lag_lds_bytes(bool has_w, int D) {
+    return sizeof(double2) * (size_t)D * LAG_T + (size_t)LAG_K * LAG_T * (16 + 16 + 16 + (has_w ? 16 : 0)) +
+           sizeof(int) * (LAG_DEC + 4);
 }
 
 // Publishes *flag = v to other workgroups after every store of this thread (and, through a
 // preceding workgroup barrier, of its workgroup): a release fence, whose L2 write-back is waited
-// for explicitly, then a relaxed store.  (The fused release store was compiled without that wait
-// in the walker's loop, so the flag could overtake the write-back.)
+// for explicitly, then a relaxed store.
 __device__ __forceinline__ void publish(int *flag, int v) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __hip_atomic_store(flag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The walker's barrier: its LDS accesses complete, then s_barrier (no wait for memory loads).
+__device__ __forceinline__ void wbar() {
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
 }
 
 // One attraction term of vertex v toward pj, d * (Wk |d|) (algorithms.cpp:1055-1058), added to
@@ -88,12 +106,50 @@ __device__ __forceinline__ void attr_add(double2 pj, double2 v, double Wk, doubl
     ay = fma(dy, sc, ay);
 }
 
+// Column ids (and weights) of the first LAG_HS settled entries [e0, e1) (clamped loads).
+template <bool HAS_W>
+__device__ __forceinline__ void settled_ids(const LagArgs &a, int e0, int e1, int *c, float *w) {
+#pragma unroll
+    for (int g = 0; g < LAG_HS; ++g) {
+        const int k = max(0, min(e0 + g, e1 - 1));  // an empty row reads entry 0
+        c[g] = a.eci[k];
+        w[g] = HAS_W ? a.ew[k] : 0.0f;
+    }
+}
+
+// pre_i = sum over the settled entries [e0, e1) in CSR order - r, the first LAG_HS column ids
+// and weights given in c / w.
+template <bool HAS_W>
+__device__ __forceinline__ double2 settled_sum(const LagArgs &a, int e0, int e1, const int *c, const float *w,
+                                               double2 v, double2 r) {
+    double ax = 0.0, ay = 0.0;
+    double2 pj[LAG_HS];
+#pragma unroll
+    for (int g = 0; g < LAG_HS; ++g) pj[g] = a.pos[c[g]];
+#pragma unroll
+    for (int g = 0; g < LAG_HS; ++g)
+        if (e0 + g < e1) attr_add(pj[g], v, HAS_W ? a.W * (double)w[g] : a.W, ax, ay);
+    for (int e = e0 + LAG_HS; e < e1; ++e)  // long rows: the rest one by one
+        attr_add(a.pos[a.eci[e]], v, HAS_W ? a.W * (double)a.ew[e] : a.W, ax, ay);
+    return make_double2(ax - r.x, ay - r.y);
+}
+
 template <bool HAS_W>
 __device__ void lag_helper(const LagArgs &a, int hid, int nhelp) {
     __shared__ int ok;
     const int t = threadIdx.x;
     for (int b = hid; b < a.nb; b += nhelp) {
         __syncthreads();  // ok of the previous batch has been read
+        const int b0 = b * a.B, bc = min(a.B, a.n - b0);
+        const int i = b0 + min(t, bc - 1);
+        // what does not change during the walk, before the wait; batch b has not moved (the
+        // walker decides whether to use this batch's sums several batches before it)
+        const double2 v = a.pos[i];
+        const double2 r = a.rep[i];
+        const int e0 = a.erp[i], e1 = a.esp[i];
+        int c[LAG_HS];
+        float w[LAG_HS];
+        settled_ids<HAS_W>(a, e0, e1, c, w);
         const int need = b - a.D + 1;  // batches [0, need) must be final
         if (need > 0) {
             if (t == 0) {
@@ -115,35 +171,11 @@ __device__ void lag_helper(const LagArgs &a, int hid, int nhelp) {
             __syncthreads();
             if (ok < 0) return;  // the walker forms the remaining batches itself
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            // the walker reads a batch's flag three batches ahead: one it has (nearly) reached is
-            // formed by the walker
-            if (ok > b - 3) continue;
+            // the walker reads a batch's flag at least LAG_LEAD batches ahead: a batch it has
+            // (nearly) reached is formed by the walker
+            if (ok > b - LAG_LEAD - 2) continue;
         }
-        const int b0 = b * a.B, bc = min(a.B, a.n - b0);
-        if (t < bc) {
-            const int i = b0 + t;
-            const double2 v = a.pos[i];
-            const int e0 = a.erp[i], e1 = a.esp[i];
-            double ax = 0.0, ay = 0.0;
-            for (int e = e0; e < e1; e += 4) {
-                int j[4];
-                double wk[4];
-                double2 pj[4];
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int k = min(e + g, e1 - 1);
-                    j[g] = a.eci[k];
-                    wk[g] = HAS_W ? a.W * (double)a.ew[k] : a.W;
-                }
-#pragma unroll
-                for (int g = 0; g < 4; ++g) pj[g] = a.pos[j[g]];
-#pragma unroll
-                for (int g = 0; g < 4; ++g)
-                    if (e + g < e1) attr_add(pj[g], v, wk[g], ax, ay);
-            }
-            const double2 r = a.rep[i];
-            a.pre[i] = make_double2(ax - r.x, ay - r.y);
-        }
+        if (t < bc) a.pre[i] = settled_sum<HAS_W>(a, e0, e1, c, w, v, r);
         // every wave's stores have reached the L2 before thread 0's release writes it back
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -153,209 +185,137 @@ __device__ void lag_helper(const LagArgs &a, int hid, int nhelp) {
 
 template <bool HAS_W>
 __device__ void lag_walker(const LagArgs &a, unsigned char *raw) {
-    constexpr int T = LAG_T, S = LAG_S, C = LAG_CAP;
-    double2 *ring = reinterpret_cast<double2 *>(raw);       // [3][T] moved positions of batch c (slot c % 3)
-    double2 *spos = ring + 3 * T;                           // [2][C] staged neighbour positions
-    double2 *sv = spos + 2 * C;                             // [2][T] staged own positions
-    double2 *sr = sv + 2 * T;                               // [2][T] staged pre_i (flagged) or rep'_i
-    double *ering = reinterpret_cast<double *>(sr + 2 * T); // [2][T] energy terms
-    int *scid = reinterpret_cast<int *>(ering + 2 * T);     // [2][C] staged column ids
-    int *srs = scid + 2 * C;                                // [2][T] first staged entry of the row
-    int *ssp = srs + 2 * T;                                 // [2][T] first recent entry
-    int *sre = ssp + 2 * T;                                 // [2][T] end of the row
-    int *sflag = sre + 2 * T;                               // [2] the staged batch was flagged
-    int *sdec = sflag + 2;                                  // [2] flags of batches b+2, b+3 (slot batch & 1)
-    float *sw = reinterpret_cast<float *>(sflag + 4);       // [2][C] staged weights (HAS_W)
-    const int n = a.n, B = a.B, nb = a.nb;
-    if (threadIdx.x < T) {  // ---- computing threads ----
-        const int t = threadIdx.x;
-        __syncthreads();  // batch 0 staged
+    constexpr int T = LAG_T, K = LAG_K;
+    const int n = a.n, B = a.B, nb = a.nb, D = a.D;
+    double2 *ring = reinterpret_cast<double2 *>(raw);     // [D][T] moved positions of batch c (slot c % D)
+    double2 *sv = ring + (size_t)D * T;                   // [K][T] staged own positions
+    double2 *sp = sv + K * T;                             // [K][T] staged pre_i (flagged) or rep'_i
+    int4 *so = reinterpret_cast<int4 *>(sp + K * T);      // [K][T] staged ring offsets of the recent neighbours
+    float4 *sw = reinterpret_cast<float4 *>(so + K * T);  // [K][T] their weights (HAS_W)
+    int *dec = reinterpret_cast<int *>(HAS_W ? (void *)(sw + K * T) : (void *)sw);  // [LAG_DEC] flags used
+    int *shelp = dec + LAG_DEC;                           // batches formed from the helpers' sums
+    const int tid = threadIdx.x;
+    if (tid < T) {  // ---- computing waves ----
+        const int t = tid;
+        int helped = 0;
+        wbar();  // batch 0 staged (and the decisions cleared)
         for (int b = 0; b < nb; ++b) {
-            const int b0 = b * B, b1 = min(n, b0 + B);
-            const int m1 = b0 - B, m2 = b0 - 2 * B;  // first vertices of batches b-1 and b-2
-            const double2 *ring1 = ring + ((b + 2) % 3) * T, *ring2 = ring + ((b + 1) % 3) * T;
-            const int cur = b & 1;
-            const int i = b0 + t;
-            if (i < b1) {
-                const int E0 = srs[cur * T];
-                const int rs = srs[cur * T + t], sp = ssp[cur * T + t], re = sre[cur * T + t];
-                const double2 v = sv[cur * T + t], r = sr[cur * T + t];
-                double px, py;
-                if (sflag[cur]) {
-                    px = r.x;
-                    py = r.y;
-                } else {  // settled neighbours are final in memory (batches <= b - D < b - 2)
-                    double ax = 0.0, ay = 0.0;
-                    for (int e = rs; e < sp; ++e) {
-                        const int k = cur * C + e - E0;
-                        attr_add(spos[k], v, HAS_W ? a.W * (double)sw[k] : a.W, ax, ay);
-                    }
-                    px = ax - r.x;
-                    py = ay - r.y;
+            const int s = (b % K) * T + t;
+            const int b0 = b * B, i = b0 + t;
+            const bool flagged = dec[b % LAG_DEC] != 0;
+            helped += flagged;
+            if (i < min(n, b0 + B)) {
+                const double2 v = sv[s];
+                double2 p = sp[s];
+                if (!flagged) {  // the helper was late (or the batch is one of the first): form pre_i
+                    const int e0 = a.erp[i], e1 = a.esp[i];
+                    int c[LAG_HS];
+                    float w[LAG_HS];
+                    settled_ids<HAS_W>(a, e0, e1, c, w);
+                    p = settled_sum<HAS_W>(a, e0, e1, c, w, v, p);
                 }
-                for (int e = sp; e < re; ++e) {  // recent neighbours: the last two batches from the ring
-                    const int k = cur * C + e - E0;
-                    const int j = scid[k];
-                    double2 pj = spos[k];
-                    if (j >= m2 && j < b0) pj = j >= m1 ? ring1[j - m1] : ring2[j - m2];
-                    attr_add(pj, v, HAS_W ? a.W * (double)sw[k] : a.W, px, py);
-                }
-                const double mag = sqrt(px * px + py * py);  // x_i += (F/|F|) * STEP, :1062-1066
+                const int4 ro = so[s];
+                float4 rw = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (HAS_W) rw = sw[s];
+                const int roff[LAG_RS] = {ro.x, ro.y, ro.z, ro.w};
+                const float rwk[LAG_RS] = {rw.x, rw.y, rw.z, rw.w};
+#pragma unroll
+                for (int k = 0; k < LAG_RS; ++k)  // recent neighbours, from the ring, in CSR order
+                    if (roff[k] >= 0) attr_add(ring[roff[k]], v, HAS_W ? a.W * (double)rwk[k] : a.W, p.x, p.y);
+                const double mag = sqrt(p.x * p.x + p.y * p.y);  // x_i += (F/|F|) * STEP, :1062-1066
                 double2 q = v;
-                q.x = q.x + (px / mag) * a.step;
-                q.y = q.y + (py / mag) * a.step;
-                ring[(b % 3) * T + t] = q;
-                ering[cur * T + t] = px * px + py * py;
+                q.x = q.x + (p.x / mag) * a.step;
+                q.y = q.y + (p.y / mag) * a.step;
+                ring[(b % D) * T + t] = q;
+                a.pos[i] = q;
+                a.eterm[i] = p.x * p.x + p.y * p.y;
             }
-            __syncthreads();  // end of batch b
+            // the stores of batch b-3 are complete (two per batch, in order; the control wave
+            // publishes them after a barrier)
+            __asm__ volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            wbar();  // end of batch b
         }
-        __syncthreads();  // the loading threads' last stores are complete
+        if (t == 0) *shelp = helped;
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        wbar();
         return;
     }
-    // ---- loading threads ----
-    const int t = threadIdx.x - T;
-    // uniform values loaded with vector loads (a scalar load shares its wait counter with LDS)
-    auto vload = [&](const int *p, int idx) {
-        int v;
-        __asm__("v_mov_b32 %0, %1" : "=v"(v) : "s"(idx));
-        return p[v];
-    };
-    struct Range {
-        int e0, e1, r0, r1;  // the batch's entries in the early and in the recent lists
-    };
-    auto range_of = [&](int bb) {
-        const int ia = min(bb * B, n), ie = min(n, bb * B + B);
-        Range R;
-        R.e0 = vload(a.erp, ia);
-        R.e1 = vload(a.erp, ie);
-        R.r0 = vload(a.rrp, ia);
-        R.r1 = vload(a.rrp, ie);
-        return R;
-    };
-    auto flag_of = [&](int bb) {
-        if (bb >= nb) return 0;
-        int idx;
-        __asm__("v_mov_b32 %0, %1" : "=v"(idx) : "s"(bb));
-        return (int)(__hip_atomic_load(a.ready + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.gen);
-    };
-    // this thread's staged entries of a batch: [x0, x1) of the list (recent or early)
-    auto load_cids = [&](bool flagged, int x0, int x1, int *c, float *w) {
-        x0 = __builtin_amdgcn_readfirstlane(x0);
-        x1 = __builtin_amdgcn_readfirstlane(x1);
-        const int *ci = flagged ? a.rci : a.eci;
-        const float *wi = flagged ? a.rw : a.ew;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int k = max(0, min(x0 + t + s * T, x1 - 1));
-            c[s] = ci[k];
-            if (HAS_W) w[s] = wi[k];
-        }
-    };
-    int cA[S], cB[S];
-    float wA[S], wB[S];
-    double pAx[S], pAy[S];
-    // flags of batches b+1 and b+2 (the path each is staged with).  A batch's flag is read once, by
-    // loading thread 0, and reaches the other loading waves through LDS after a barrier: lanes that
-    // read it themselves could see it change while a helper publishes and stage one batch two ways.
-    bool fA, fB;
-    Range R2, R3;        // entry ranges of batches b+2, b+3
-    // stage batch bb (flag fl) into half `half`
-    auto stage = [&](int bb, int half, bool fl) {
-        const int i = min(bb * B + t, n - 1);
-        int rs, sp, re;
-        if (fl) {
-            rs = a.rrp[i];
-            re = a.rrp[i + 1];
-            sp = rs;
-        } else {
-            rs = a.erp[i];
-            sp = a.esp[i];
-            re = a.erp[i + 1];
-        }
-        const double2 v = a.pos[i], r = fl ? a.pre[i] : a.rep[i];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int k = half * C + t + s * T;
-            spos[k] = make_double2(pAx[s], pAy[s]);
-            scid[k] = cA[s];
-            if (HAS_W) sw[k] = wA[s];
-        }
-        srs[half * T + t] = rs;
-        ssp[half * T + t] = sp;
-        sre[half * T + t] = re;
-        sv[half * T + t] = v;
-        sr[half * T + t] = r;
-        if (t == 0) sflag[half] = fl;
-    };
-    auto gather = [&]() {
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const double2 q = a.pos[cA[s]];
-            pAx[s] = q.x;
-            pAy[s] = q.y;
-        }
-    };
-    int helped = 0;  // batches staged with the helpers' sums (loading thread 0)
-    {
-        // batches 0, 1 and 2 are formed by the walker
-        const Range R0 = range_of(0), R1 = range_of(1);
-        R2 = range_of(2);
-        fA = false;
-        if (t == 0) sdec[0] = 0;  // batch 2
-        load_cids(false, R0.e0, R0.e1, cA, wA);
-        gather();
-        stage(0, 0, false);
-        load_cids(false, R1.e0, R1.e1, cA, wA);
-    }
-    __syncthreads();  // batch 0 staged
-    for (int b = 0; b < nb; ++b) {
-        const bool f2 = sdec[b & 1] != 0;  // flag of batch b+2, read during batch b-1
-        // acquire before reading anything a helper wrote (batch b+1 was flagged when its flag
-        // was read; every load of the previous batch has completed, so the fence waits for none)
-        if (fA) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // the stores of batches [0, b-1) completed before the last barrier: publish them
-        if (t == 0 && b >= 2 && (b - 1) % a.RK == 0) publish(a.done, b - 1);
-        if (b > 0) {  // the moves of batch b-1 to memory
-            const int c0 = (b - 1) * B;
-            if (t < B && c0 + t < n) {
-                a.pos[c0 + t] = ring[((b + 2) % 3) * T + t];
-                a.eterm[c0 + t] = ering[((b + 1) & 1) * T + t];
+    if (tid < 2 * T) {  // ---- loading waves: batch b + 3 during batch b ----
+        const int t = tid - T, w = t >> 6;
+        constexpr int OPS = HAS_W ? 4 : 3;  // loads per batch and wave
+        // one batch into its slot; pre_i if the control wave recorded a usable flag, else rep'_i
+        auto issue = [&](int x, bool use_dec) {
+            const int slot = (x % K) * T + w * 64;  // this wave's 64 vertices of the slot
+            const int i = min(x * B + t, n - 1);
+            int f = 0;
+            if (use_dec) {  // an LDS read the compiler does not order after the loads in flight
+                const lds_void *dp = (const lds_void *)(dec + x % LAG_DEC);
+                __asm__ volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(f) : "v"(dp) : "memory");
             }
+            __builtin_amdgcn_global_load_lds((const void *)(a.pos + i), (lds_void *)(sv + slot), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(f ? a.pre + i : a.rep + i), (lds_void *)(sp + slot), 16,
+                                             0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(a.rofs + i), (lds_void *)(so + slot), 16, 0, 0);
+            if (HAS_W) __builtin_amdgcn_global_load_lds((const void *)(a.rw + i), (lds_void *)(sw + slot), 16, 0, 0);
+        };
+        // the loads of batch x have landed, given that batches x+1 .. last were issued after it
+        auto land = [&](int x, int last) {
+            const int younger = max(0, last - x);
+            if (younger >= 2) {
+                if (OPS == 4) __asm__ volatile("s_waitcnt vmcnt(8)" ::: "memory");
+                else __asm__ volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            } else if (younger == 1) {
+                if (OPS == 4) __asm__ volatile("s_waitcnt vmcnt(4)" ::: "memory");
+                else __asm__ volatile("s_waitcnt vmcnt(3)" ::: "memory");
+            } else {
+                __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            }
+        };
+        // prologue: batches 0..2 unflagged (before the control wave's first barrier)
+        for (int x = 0; x < K - 1 && x < nb; ++x) issue(x, false);
+        land(0, min(K - 2, nb - 1));
+        wbar();
+        for (int b = 0; b < nb; ++b) {
+            const int x = b + K - 1;  // slot (b-1) % K: the computing waves left it at the last barrier
+            if (x < nb) issue(x, true);
+            if (b + 1 < nb) land(b + 1, min(x, nb - 1));
+            wbar();  // end of batch b
         }
-        gather();  // batch b+1: positions of its staged entries
-        fB = f2;   // batch b+2: path, column ids
-        load_cids(fB, fB ? R2.r0 : R2.e0, fB ? R2.r1 : R2.e1, cB, wB);
-        if (t == 0) sdec[(b + 1) & 1] = flag_of(b + 3);  // batch b+3: flag, ranges
-        R3 = range_of(b + 3);
-        stage(b + 1, (b + 1) & 1, fA);
-        helped += (fA && b + 1 < nb) ? 1 : 0;
-        // the stores of batch b-1 are complete before the barrier (gathers after it read them)
-        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();  // end of batch b
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            cA[s] = cB[s];
-            if (HAS_W) wA[s] = wB[s];
+        wbar();
+        return;
+    }
+    // ---- control wave ----
+    const int l = tid - 2 * T;
+    if (l < LAG_DEC) dec[l] = 0;  // the first batches are formed by the walker
+    int val = 0;
+    wbar();
+    for (int b = 0; b < nb; ++b) {
+        if ((b & 1) == 0) {
+            if (b >= 2) {  // the flags read two batches ago, of batches b - 2 + LAG_LEAD + l
+                const int xx = b - 2 + LAG_LEAD + l;
+                const bool good = l < 2 && xx < nb && val == a.gen;
+                if (__ballot(good)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                // batch xx is staged during batch xx - 3 >= b + 1, after this batch's barrier
+                if (l < 2 && xx < nb) dec[xx % LAG_DEC] = good ? 1 : 0;
+                // batches [0, b - 3): the computing waves completed batch b - 4's stores before
+                // the last barrier
+                if (l == 0 && b >= 4) publish(a.done, b - 3);
+            }
+            val = 0;
+            const int y = b + LAG_LEAD + l;
+            if (l < 2 && y < nb) val = __hip_atomic_load(a.ready + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        fA = fB;
-        R2 = R3;
+        wbar();  // end of batch b
     }
-    // the moves of the last batch, then every batch is final: the helpers may stop
-    const int c0 = (nb - 1) * B;
-    if (t < B && c0 + t < n) {
-        a.pos[c0 + t] = ring[((nb - 1) % 3) * T + t];
-        a.eterm[c0 + t] = ering[((nb - 1) & 1) * T + t];
-    }
-    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    if (t == 0) {
-        a.done[1] = helped;
-        publish(a.done, nb);
+    wbar();
+    if (l == 0) {
+        a.done[1] = *shelp;
+        publish(a.done, nb);  // every batch is final: the helpers may stop
     }
 }
 
 template <bool HAS_W>
-__global__ __launch_bounds__(2 * LAG_T) void k_walk_lag(LagArgs a) {
+__global__ __launch_bounds__(LAG_WG) void k_walk_lag(LagArgs a) {
     extern __shared__ __align__(16) unsigned char lag_raw[];
     if (__builtin_amdgcn_readfirstlane(*a.status)) return;
     if (blockIdx.x == 0)
@@ -375,76 +335,72 @@ bool lag_walk_fits(bl_graph *g, int B, int D) {
     g->lag_ok = false;
     const int n = g->n;
     const int nb = (n + B - 1) / B;
-    if (!(B <= LAG_T && D >= 3 && nb >= 2 * D && g->nnz > 0 && g->max_row <= LAG_MAXDEG &&
-          (int)g->h_rowptr.size() == n + 1 && (int64_t)g->h_colids.size() == g->nnz))
+    if (!(B <= LAG_T && D >= 4 && D <= LAG_DMAX && nb >= 2 * D && nb >= LAG_LEAD + 4 && g->nnz > 0 &&
+          g->max_row <= LAG_MAXDEG && (int)g->h_rowptr.size() == n + 1 && (int64_t)g->h_colids.size() == g->nnz))
         return false;
     const bool hw = !g->h_w.empty();
-    std::vector<int32_t> erp(n + 1, 0), esp(n, 0), rrp(n + 1, 0), eci, rci;
-    std::vector<float> ew, rw;
+    std::vector<int32_t> erp(n + 1, 0), esp(n, 0), eci;
+    std::vector<float> ew;
+    std::vector<int32_t> rofs(4 * (size_t)n, -1);
+    std::vector<float> rw(hw ? 4 * (size_t)n : 0, 0.0f);
     eci.reserve(g->nnz / 2 + 1);
     for (int i = 0; i < n; ++i) {
         const int bi = i / B;
+        int nr = 0;
         for (int pass = 0; pass < 2; ++pass) {  // settled first, then recent, each in CSR order
             if (pass == 1) esp[i] = (int32_t)eci.size();
             for (int e = g->h_rowptr[i]; e < g->h_rowptr[i + 1]; ++e) {
-                const int bj = g->h_colids[e] / B;
+                const int j = g->h_colids[e], bj = j / B;
                 if (bj >= bi) continue;  // late: summed by k_late_attract
                 const bool settled = bj <= bi - D;
                 if (settled != (pass == 0)) continue;
-                eci.push_back(g->h_colids[e]);
+                eci.push_back(j);
                 if (hw) ew.push_back(g->h_w[e]);
                 if (!settled) {
-                    rci.push_back(g->h_colids[e]);
-                    if (hw) rw.push_back(g->h_w[e]);
+                    if (nr == LAG_RS) return false;  // more recent neighbours than the stage holds
+                    rofs[4 * (size_t)i + nr] = (bj % D) * LAG_T + (j - bj * B);
+                    if (hw) rw[4 * (size_t)i + nr] = g->h_w[e];
+                    ++nr;
                 }
             }
         }
         erp[i + 1] = (int32_t)eci.size();
-        rrp[i + 1] = (int32_t)rci.size();
     }
-    for (int b0 = 0; b0 < n; b0 += B)
-        if (erp[std::min(n, b0 + B)] - erp[b0] > LAG_CAP) return false;
-    // the loading threads read index 0 of an empty list (clamped loads): keep one valid entry
+    // the clamped loads read index 0 of an empty list: keep one valid entry
     if (eci.empty()) eci.push_back(0), ew.push_back(0.0f);
-    if (rci.empty()) rci.push_back(0), rw.push_back(0.0f);
     if (g->lag_erp.ensure(4 * (size_t)(n + 1)) || g->lag_esp.ensure(4 * (size_t)n) ||
-        g->lag_rrp.ensure(4 * (size_t)(n + 1)) || g->lag_eci.ensure(4 * eci.size()) ||
-        g->lag_rci.ensure(4 * rci.size()) || g->lag_pre.ensure(16 * (size_t)n) ||
+        g->lag_eci.ensure(4 * eci.size()) || g->lag_rofs.ensure(16 * (size_t)n) || g->lag_pre.ensure(16 * (size_t)n) ||
         g->lag_ready.ensure(4 * (size_t)nb + 256))
         return false;
-    if (hw && (g->lag_ew.ensure(4 * ew.size()) || g->lag_rw.ensure(4 * rw.size()))) return false;
+    if (hw && (g->lag_ew.ensure(4 * ew.size()) || g->lag_rw.ensure(16 * (size_t)n))) return false;
     if (hipMemcpy(g->lag_erp.p, erp.data(), 4 * (size_t)(n + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(g->lag_esp.p, esp.data(), 4 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->lag_rrp.p, rrp.data(), 4 * (size_t)(n + 1), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(g->lag_eci.p, eci.data(), 4 * eci.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->lag_rci.p, rci.data(), 4 * rci.size(), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(g->lag_rofs.p, rofs.data(), 16 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
         return false;
     if (hw && (hipMemcpy(g->lag_ew.p, ew.data(), 4 * ew.size(), hipMemcpyHostToDevice) != hipSuccess ||
-               hipMemcpy(g->lag_rw.p, rw.data(), 4 * rw.size(), hipMemcpyHostToDevice) != hipSuccess))
+               hipMemcpy(g->lag_rw.p, rw.data(), 16 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess))
         return false;
-    // flags hold launch generations (never 0); done at the end of the ready array
+    // flags hold launch generations (never 0); done (and the helped count) after the flags
     if (hipMemset(g->lag_ready.p, 0, 4 * (size_t)nb + 256) != hipSuccess) return false;
     g->lag_gen = 0;
     g->lag_ok = true;
     return true;
 }
 
-size_t lag_walk_lds(bool has_w) { return lag_lds(has_w); }
-
 // the count of batches the last lagged walk staged with the helpers' sums
 int *lag_walk_helped(bl_graph *g) { return g->lag_ready.as<int>() + (g->n + g->lag_batch - 1) / g->lag_batch + 33; }
 
 // One iteration's walk (after k_late_attract has folded the late attraction into the repulsion).
-int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, int rk, hipStream_t s) {
+int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, hipStream_t s) {
     const int n = g->n, B = p->batch;
     LagArgs a;
     a.erp = g->lag_erp.as<int>();
     a.esp = g->lag_esp.as<int>();
     a.eci = g->lag_eci.as<int>();
     a.ew = g->has_w ? g->lag_ew.as<float>() : nullptr;
-    a.rrp = g->lag_rrp.as<int>();
-    a.rci = g->lag_rci.as<int>();
-    a.rw = g->has_w ? g->lag_rw.as<float>() : nullptr;
+    a.rofs = g->lag_rofs.as<int4>();
+    a.rw = g->has_w ? g->lag_rw.as<float4>() : nullptr;
     a.W = p->weight;
     a.pos = pos;
     a.rep = g->bh.rep.as<double2>();
@@ -457,7 +413,6 @@ int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, 
     a.n = n;
     a.B = B;
     a.D = g->lag_d;
-    a.RK = std::max(1, rk);
     a.step = step;
     a.eterm = g->eterm.as<double>();
     a.status = g->bh.misc.as<int>();
@@ -466,17 +421,17 @@ int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, 
         khz = 100000;
     a.ticks = (long long)khz * 1000;  // 1 s
     BL_HIP(hipMemsetAsync(a.done, 0, 2 * sizeof(int), s));
-    const size_t lds = lag_lds(g->has_w);
+    const size_t lds = lag_lds_bytes(g->has_w, a.D);
     if (g->has_w)
-        hipLaunchKernelGGL(k_walk_lag<true>, dim3(1 + helpers), dim3(2 * LAG_T), lds, s, a);
+        hipLaunchKernelGGL(k_walk_lag<true>, dim3(1 + helpers), dim3(LAG_WG), lds, s, a);
     else
-        hipLaunchKernelGGL(k_walk_lag<false>, dim3(1 + helpers), dim3(2 * LAG_T), lds, s, a);
+        hipLaunchKernelGGL(k_walk_lag<false>, dim3(1 + helpers), dim3(LAG_WG), lds, s, a);
     BL_HIP(hipGetLastError());
     return BL_OK;
 }
 
 int lag_walk_prepare(bool has_w) {
-    const int lb = (int)lag_lds(has_w);
+    const int lb = (int)lag_lds_bytes(has_w, LAG_DMAX);
     BL_HIP(hipFuncSetAttribute(has_w ? reinterpret_cast<const void *>(k_walk_lag<true>)
                                      : reinterpret_cast<const void *>(k_walk_lag<false>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, lb));

@@ -269,12 +269,12 @@ def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, w
     one-workgroup row walk (partial sums in another order: 1e-10 of the diameter after three
     iterations), and bit for bit against itself with no helpers at all and with other lags and
     publishing periods: whichever workgroup formed a batch's sum, it is the same."""
-    if kind == "grid":
-        rp, ci = graphs.csr_from_edges(*graphs.grid(128, 128))
+    if kind == "grid":  # natural order: every early neighbour is in the previous batch
+        rp, ci = graphs.csr_from_edges(*graphs.grid(160, 160))
     elif kind == "delaunay":
-        rp, ci = graphs.csr_from_edges(*graphs.delaunay_holes(40_000, seed=3))
-    else:
-        rp, ci = graphs.csr_from_edges(*graphs.rgg(60_000, seed=5))
+        rp, ci = graphs.csr_from_edges(*graphs.delaunay_holes(200_000, seed=3))
+    else:  # at most 4 neighbours of a vertex in the 19 batches before its own (lag 20)
+        rp, ci = graphs.csr_from_edges(*graphs.rgg(250_000, seed=5))
     n = len(rp) - 1
     w = np.random.default_rng(7).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
     x0 = np.random.default_rng(8).random((n, 2))
@@ -282,9 +282,9 @@ def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, w
     out, helped = {}, {}
     # a short lag has the helpers read positions the walker has just published
     for mode, env in (("row", dict(BL_WALK="row")), ("lag", {}), ("lag_h0", dict(BL_LAG_H="0")),
-                      ("lag_rk1", dict(BL_LAG_RK="1")), ("lag_d8", dict(BL_LAG_D="8", BL_LAG_RK="1")),
-                      ("lag_d8_h0", dict(BL_LAG_D="8", BL_LAG_H="0"))):
-        for k in ("BL_WALK", "BL_LAG_H", "BL_LAG_RK", "BL_LAG_D"):
+                      ("lag_h4", dict(BL_LAG_H="4")), ("lag_d12", dict(BL_LAG_D="12")),
+                      ("lag_d12_h0", dict(BL_LAG_D="12", BL_LAG_H="0"))):
+        for k in ("BL_WALK", "BL_LAG_H", "BL_LAG_D"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -299,10 +299,27 @@ def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, w
     # partial sums in another order: rounding differences, grown over three chaotic iterations
     assert np.abs(out["lag"] - out["row"]).max() <= 1e-9 * d
     assert abs(out["lag_E"] - out["row_E"]) <= 1e-9 * out["row_E"]
-    assert helped["lag_h0"] == 0 and helped["lag_d8_h0"] == 0
-    if kind != "grid":  # on the grid every early neighbour is recent: the helpers' sums are trivial
-        assert helped["lag"] > nb // 2 and helped["lag_d8"] > 0
+    assert helped["lag_h0"] == 0 and helped["lag_d12_h0"] == 0
+    assert helped["lag"] > nb // 2 and helped["lag_d12"] > 0
     assert np.array_equal(out["lag"], out["lag_h0"]) and out["lag_E"] == out["lag_h0_E"]
-    assert np.array_equal(out["lag"], out["lag_rk1"]) and out["lag_E"] == out["lag_rk1_E"]
-    assert np.array_equal(out["lag_d8"], out["lag_d8_h0"]) and out["lag_d8_E"] == out["lag_d8_h0_E"]
-    assert np.abs(out["lag_d8"] - out["row"]).max() <= 1e-9 * d
+    assert np.array_equal(out["lag"], out["lag_h4"]) and out["lag_E"] == out["lag_h4_E"]
+    assert np.array_equal(out["lag_d12"], out["lag_d12_h0"]) and out["lag_d12_E"] == out["lag_d12_h0_E"]
+    assert np.abs(out["lag_d12"] - out["row"]).max() <= 1e-9 * d
+
+
+def test_lag_walk_declines_crowded_recent_batches(bl, monkeypatch):
+    """A vertex with more than four neighbours in the batches just before its own (40k-vertex
+    Delaunay graph, batches of 256, lag 20) does not fit the walk's stage: the row walk runs, and
+    a shorter lag that fits takes the lagged walk again."""
+    rp, ci = graphs.csr_from_edges(*graphs.delaunay_holes(40_000, seed=3))
+    x0 = np.random.default_rng(8).random((len(rp) - 1, 2))
+    for k in ("BL_WALK", "BL_LAG_H", "BL_LAG_D"):
+        monkeypatch.delenv(k, raising=False)
+    with bl.Graph(rp, ci) as g:
+        out, st = g.run(x0, algo=4, iters=1, batch=256, bht=1.2)
+        assert st.walk == bl.WALK_ROW
+    monkeypatch.setenv("BL_LAG_D", "6")
+    with bl.Graph(rp, ci) as g:
+        out6, st = g.run(x0, algo=4, iters=1, batch=256, bht=1.2)
+        assert st.walk == bl.WALK_LAG
+    assert np.abs(out6 - out).max() <= 1e-11 * diameter(out)
