@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libbatchlayout.so")
 
 BL_OK = 0
 STATUS = {0: "BL_OK", -1: "BL_E_ARG", -2: "BL_E_IO", -3: "BL_E_FORMAT", -4: "BL_E_DEVICE",
-          -5: "BL_E_DUPKEY", -6: "BL_E_NOMEM", -7: "BL_E_GRAPH"}
+          -5: "BL_E_DUPKEY", -6: "BL_E_NOMEM", -7: "BL_E_GRAPH", -8: "BL_E_SCRATCH"}
 ALGO_CACHE, ALGO_LINLOG, ALGO_BH, ALGO_BHCACHE, ALGO_FA = 2, 3, 4, 6, 8
 # physically consistent Barnes-Hut layouts (no reference counterpart)
 ALGO_FAR_MONO, ALGO_FAR_EXACT, ALGO_FAR_LEARNED, ALGO_FAR_QUAD = 40, 41, 42, 43
@@ -69,6 +69,8 @@ def lib():
         L.bl_graph_size.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.bl_run.argtypes = [C.c_void_p, _f64p, C.POINTER(bl_params), C.POINTER(bl_stats)]
         L.bl_run_device.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(bl_params), C.POINTER(bl_stats), C.c_void_p]
+        L.bl_run_device_static.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(bl_params), C.POINTER(bl_stats),
+                                           C.c_void_p]
         L.bl_mtx_read.argtypes = [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                   C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_int32))]
         L.bl_free.argtypes = [C.c_void_p]
@@ -185,6 +187,15 @@ class Graph:
         st = bl_stats()
         _chk(lib().bl_run_device(self._h, C.c_void_p(d_xy_ptr), C.byref(p), C.byref(st),
                                  None if stream is None else C.c_void_p(stream)), "bl_run_device")
+        return _stats(st)
+
+    def run_device_static(self, d_xy_ptr: int, stream: int | None = None, **kw) -> Stats:
+        """run_device for algos 2, 3 and 8 with the workspace in the library's static device
+        array: the call allocates no device memory (BLError BL_E_SCRATCH when it does not fit)."""
+        p = params(**kw)
+        st = bl_stats()
+        _chk(lib().bl_run_device_static(self._h, C.c_void_p(d_xy_ptr), C.byref(p), C.byref(st),
+                                        None if stream is None else C.c_void_p(stream)), "bl_run_device_static")
         return _stats(st)
 
 

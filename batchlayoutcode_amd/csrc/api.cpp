@@ -275,6 +275,37 @@ extern "C" int bl_run_device(bl_graph *g, double *d_xy, const bl_params *p, bl_s
     return rc;
 }
 
+// The minibatch family with its workspace in the static device array: no allocation, no
+// profiling events (both are refused up front rather than done quietly).
+extern "C" int bl_run_device_static(bl_graph *g, double *d_xy, const bl_params *p, bl_stats *st, void *stream) {
+    if (!g || !d_xy) return BL_E_ARG;
+    int rc = validate_params(p);
+    if (rc) return rc;
+    if (p->profile) return BL_E_ARG;
+    int law;
+    double decay;
+    switch (p->algo) {
+        case BL_ALGO_CACHE: law = LAW_FR; decay = 0.999; break;
+        case BL_ALGO_LINLOG: law = LAW_LINLOG; decay = 0.99; break;
+        case BL_ALGO_FA: law = LAW_FA; decay = 0.99; break;
+        default: return BL_E_ARG;
+    }
+    bl_stats local;
+    memset(&local, 0, sizeof(local));
+    hipStream_t own = g->stream;
+    if (stream) g->stream = static_cast<hipStream_t>(stream);
+    const auto t0 = std::chrono::steady_clock::now();
+    rc = run_minibatch(g, d_xy, p, law, decay, p->step0, p->iters, p->algo == BL_ALGO_CACHE && p->converged != 0,
+                       &local, true);
+    if (hipStreamSynchronize(g->stream) != hipSuccess && rc == BL_OK) rc = BL_E_DEVICE;
+    const auto t1 = std::chrono::steady_clock::now();
+    g->stream = own;
+    local.seconds = std::chrono::duration<double>(t1 - t0).count();
+    local.status = rc;
+    if (st) *st = local;
+    return rc;
+}
+
 extern "C" int bl_run(bl_graph *g, double *xy_inout, const bl_params *p, bl_stats *st) {
     if (!g || !xy_inout) return BL_E_ARG;
     int rc = validate_params(p);

@@ -209,8 +209,10 @@ inline void effective_neighbors(const int32_t *rowptr, const int32_t *colids, co
 }
 
 // minibatch.hip
+// static_ws: the workspace in the library's static device array instead of the graph's buffers
+// (allocates nothing; BL_E_SCRATCH when it does not fit)
 int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0,
-                  int iters, bool converged, bl_stats *st);
+                  int iters, bool converged, bl_stats *st, bool static_ws = false);
 // bh.hip
 // far_mode 0: the reference's unit-vector law; else a FarMode of bh_common.h
 int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta,

@@ -30,6 +30,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include <hip/hip_ext.h>
 
@@ -274,8 +275,38 @@ int launch_energy(const double *eterm, int n, double *out, int precision, hipStr
     return BL_OK;
 }
 
+// ---- the static workspace (bl_run_device_static) ------------------------------------------
+// One array per GPU, held until the process ends.  On the host its name stands for a host-side
+// placeholder, not device memory, so host code uses only the device address that
+// hipGetSymbolAddress returns, looked up once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[BL_STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+
+static int static_scratch(int device, unsigned char **out) {
+    static std::mutex mu;
+    static void *addr[MAX_DEVICES] = {};
+    if (device < 0 || device >= MAX_DEVICES) return BL_E_DEVICE;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        BL_HIP(hipGetDevice(&cur));
+        if (cur != device) BL_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return device_error(e, "hipGetSymbolAddress(scratch)", __LINE__);
+        if (!p) return BL_E_DEVICE;
+        addr[device] = p;
+    }
+    *out = static_cast<unsigned char *>(addr[device]);
+    return BL_OK;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0, int iters,
-                  bool converged, bl_stats *st) {
+                  bool converged, bl_stats *st, bool static_ws) {
     const int n = g->n;
     const int B = p->batch;
     const int nb = (n + B - 1) / B;
@@ -310,12 +341,30 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
         P = G + g->corr_max_pieces;
         part_elems = (size_t)bmax * P;
     }
-    if (g->part.ensure(sizeof(double2) * part_elems)) return BL_E_NOMEM;
-    if (g->eterm.ensure(sizeof(double) * (size_t)n)) return BL_E_NOMEM;
-    if (g->energy.ensure(sizeof(double) * (4 + ENERGY_PARTS))) return BL_E_NOMEM;
-    double2 *part = g->part.as<double2>();
-    double *eterm = g->eterm.as<double>();
-    double *energy = g->energy.as<double>();
+    double2 *part;
+    double *eterm, *energy;
+    if (static_ws) {
+        // [energy shares | partial forces | energy terms]; every word a step reads is written
+        // earlier in the same step, so the contents of the previous call do not matter
+        static_assert(sizeof(double) * (4 + ENERGY_PARTS) <= 4096, "energy shares exceed their part");
+        const size_t off_part = 4096;
+        const size_t off_eterm = off_part + round256(sizeof(double2) * part_elems);
+        const size_t need = off_eterm + round256(sizeof(double) * (size_t)n);
+        if (need > BL_STATIC_SCRATCH_BYTES) return BL_E_SCRATCH;
+        unsigned char *ws = nullptr;
+        const int rc = static_scratch(g->device, &ws);
+        if (rc) return rc;
+        energy = reinterpret_cast<double *>(ws);
+        part = reinterpret_cast<double2 *>(ws + off_part);
+        eterm = reinterpret_cast<double *>(ws + off_eterm);
+    } else {
+        if (g->part.ensure(sizeof(double2) * part_elems)) return BL_E_NOMEM;
+        if (g->eterm.ensure(sizeof(double) * (size_t)n)) return BL_E_NOMEM;
+        if (g->energy.ensure(sizeof(double) * (4 + ENERGY_PARTS))) return BL_E_NOMEM;
+        part = g->part.as<double2>();
+        eterm = g->eterm.as<double>();
+        energy = g->energy.as<double>();
+    }
 
     Prof prof;
     prof.on = p->profile != 0;

@@ -36,6 +36,7 @@ extern "C" {
 #define BL_E_DUPKEY (-5)     /* Barnes-Hut: two vertices share a Morton cell (the reference recurses forever) */
 #define BL_E_NOMEM (-6)      /* host or device allocation failed */
 #define BL_E_GRAPH (-7)      /* CSR fails validation (rowptr not monotone, colid out of range) */
+#define BL_E_SCRATCH (-8)    /* bl_run_device_static: the workspace exceeds the static device array */
 
 /* ---- algorithms (the reference's -algo values, Test/BatchLayout.cpp:132-174) ---- */
 #define BL_ALGO_CACHE 2      /* cacheBlockingminiBatchForceDirectedAlgorithm, algorithms.cpp:715-794 */
@@ -147,6 +148,23 @@ int bl_run(bl_graph *g, double *xy_inout, const bl_params *p, bl_stats *st);
  * the caller with hipMalloc); `stream` is a hipStream_t or NULL for the library's own.
  * Returns after the layout has finished. */
 int bl_run_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats *st, void *stream);
+
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * recorded under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * bl_run_device for the minibatch family (algos 2, 3, 8; profile 0, otherwise BL_E_ARG) whose
+ * workspace (per-batch partial forces, per-vertex energy terms, energy shares) lives in one
+ * static device array of BL_STATIC_SCRATCH_BYTES per GPU, which the GPU holds until the process
+ * ends, instead of the graph's buffers: the call allocates nothing (the graph handle, created
+ * beforehand, holds the CSR).  The workspace takes 4096 + 16 * min(batch, n) * P + 8 * n bytes
+ * (each part rounded up to 256), P = the j chunks of one batch step (at most
+ * max(256, n / 1024 + 1) in the FP32 and mixed modes, n / 512 rounded up in FP64 mode) plus,
+ * outside FP64 mode, the correction pieces of the longest row (one per 256 entries); batches of
+ * 256 fit up to about 2.5M vertices (2M in FP64 mode).  More than the array holds returns
+ * BL_E_SCRATCH before anything runs.  Every call on one GPU uses that GPU's array and leaves
+ * its contents behind (nothing in it needs to start at zero): calls that overlap on one GPU,
+ * from different threads or streams, race. */
+#define BL_STATIC_SCRATCH_BYTES (32u << 20)
+int bl_run_device_static(bl_graph *g, double *d_xy, const bl_params *p, bl_stats *st, void *stream);
 
 /* ---- host I/O: the reference's file formats ------------------------------------- */
 /* .mtx -> CSR, replacing SetInputMatricesAsCSR (sample/commonutility.h:44-53) =
