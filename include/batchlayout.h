@@ -125,7 +125,10 @@ typedef struct {
 #define BL_WALK_ROW 4        /* one workgroup, one thread per vertex, short rows */
 #define BL_WALK_GRID 5       /* several co-resident workgroups, one grid barrier per batch */
 #define BL_WALK_LAG 6        /* one walking workgroup (one thread per vertex, short rows) and helper
-                                workgroups that sum the neighbours moved at least 16 batches earlier */
+                                workgroups that sum the neighbours moved at least D batches earlier
+                                (D = 20 by default; chosen when B <= 256, every row has at most 64
+                                entries and no vertex has more than 4 neighbours in the D - 1
+                                batches before its own) */
 
 /* Fills *p with the reference CLI defaults. */
 int bl_params_default(bl_params *p);
