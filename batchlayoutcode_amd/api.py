@@ -44,7 +44,8 @@ class bl_stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("iters_done", C.c_int32), ("status", C.c_int32),
                 ("seconds", C.c_double), ("step_final", C.c_double), ("kernel_ms", C.c_double * 8),
                 ("kernel_launches", C.c_int32 * 8), ("walk", C.c_int32), ("far_cells", C.c_int64),
-                ("far_model_iters", C.c_int32), ("far_judge_err", C.c_float * 2), ("walk_helped", C.c_int32)]
+                ("far_model_iters", C.c_int32), ("far_judge_err", C.c_float * 2), ("walk_helped", C.c_int32),
+                ("walk_lag", C.c_int32)]
 
 
 _lib = None
@@ -123,12 +124,13 @@ class Stats:
     far_model_iters: int = 0
     far_judge_err: tuple = (0.0, 0.0)
     walk_helped: int = 0
+    walk_lag: int = 0
 
 
 def _stats(st: bl_stats) -> Stats:
     return Stats(st.energy, st.iters_done, st.seconds, st.step_final, tuple(st.kernel_ms),
                  tuple(st.kernel_launches), st.walk, st.far_cells, st.far_model_iters, tuple(st.far_judge_err),
-                 st.walk_helped)
+                 st.walk_helped, st.walk_lag)
 
 
 class Graph:

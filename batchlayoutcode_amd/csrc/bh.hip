@@ -2025,12 +2025,13 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     if (use_grid) use_row = false;
     // the lagged walk (helper workgroups) where the row walk fits and the graph has enough batches;
     // BL_LAG_D / BL_LAG_H set its lag and helper count
-    int lag_d = 20, lag_h = 32;
+    int lag_d = 0, lag_h = 32;
     if (const char *v = getenv("BL_LAG_D")) lag_d = atoi(v);
     if (const char *v = getenv("BL_LAG_H")) lag_h = std::max(0, atoi(v));
-    bool use_lag = use_row && lag_walk_fits(g, p->batch, lag_d);
-    if (const char *wenv = getenv("BL_WALK"))
-        if (!strcmp(wenv, "row")) use_lag = false;
+    if (use_row && lag_d == 0) lag_d = lag_walk_choose(g, p->batch);
+    bool use_lag = use_row && lag_d > 0 && lag_walk_fits(g, p->batch, lag_d);
+    if (const char *wenv = getenv("BL_WALK"))  // row and single: one workgroup, no helpers
+        if (!strcmp(wenv, "row") || !strcmp(wenv, "single")) use_lag = false;
     if (use_lag) {
         rc = lag_walk_prepare(g->has_w);
         if (rc) return rc;
@@ -2157,6 +2158,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         st->far_cells = (int64_t)cells;
         st->far_model_iters = model_iters;
         st->walk_helped = helped;
+        st->walk_lag = walk == BL_WALK_LAG ? lag_d : 0;
         st->far_judge_err[0] = judge_err[0];
         st->far_judge_err[1] = judge_err[1];
         st->energy = E;

@@ -33,6 +33,7 @@ int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 // batch size B and lag D (false: the walk does not take this graph and batch size);
 // launch_walk_lag runs one iteration's walk with `helpers` helper workgroups (after
 // k_late_attract); lag_walk_prepare sets the kernels' LDS limit.
+int lag_walk_choose(bl_graph *g, int B);
 bool lag_walk_fits(bl_graph *g, int B, int D);
 int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, hipStream_t s);
 int lag_walk_prepare(bool has_w);

@@ -80,6 +80,7 @@ struct bl_graph {
     // ring offsets and weights (4 per vertex), the helpers' per-vertex sums and per-batch flags
     // (launch generations)
     int32_t lag_batch = 0, lag_d = 0, lag_gen = 0;
+    int32_t lag_auto_batch = 0, lag_auto_d = 0;  // the default lag for batch size lag_auto_batch
     bool lag_ok = false;
     bl::DevBuf lag_erp, lag_esp, lag_eci, lag_ew, lag_rofs, lag_rw, lag_pre, lag_ready;
     // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),

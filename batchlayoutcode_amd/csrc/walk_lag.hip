@@ -18,6 +18,13 @@
 // published batches [0, b - D + 1), gathers the settled positions, writes pre_i of the batch to
 // memory and flags the batch with the launch's generation number.
 //
+// What one workgroup writes for another (moved positions, pre_i) is stored with agent-scope
+// stores, which the XCD's L2 writes through to memory: once such a store has completed
+// (s_waitcnt), a relaxed flag store issued after it publishes it, and a reader on another XCD
+// that sees the flag and then acquires (invalidating its own L2's stale lines) reads the new
+// value.  A release fence instead writes back the whole L2 and took about 0.6 us, every second
+// batch, on the walker's critical path.
+//
 // Workgroup 0, the walker, never waits for a helper:
 //   computing waves (LAG_T threads, thread t owns vertex b0 + t of every batch) read only LDS: a
 //     staged batch (own position, pre_i or rep'_i, the ring slots of the recent neighbours) and a
@@ -29,8 +36,11 @@
 //     wait only for the loads of batch b + 1, so three batches of loads are in flight;
 //   a control wave reads the helpers' flags two batches at a time, LAG_LEAD batches ahead,
 //     acquires, and records for each batch, before the batch is staged, whether its pre_i is used;
-//     every second batch it publishes the walker's progress to the helpers (batches whose stores
+//     every batch it publishes the walker's progress to the helpers (batches whose stores
 //     completed: the computing waves wait for the stores of three batches back).
+// The computing waves' arithmetic is FP64 throughout; |d| and the unit vector F/|F| come from
+// the hardware reciprocal square root refined by two Newton steps (a few ulp from the IEEE
+// square root and division the row walk uses).
 // Barriers are raw s_barrier with explicit LDS waits (a workgroup barrier would also wait for the
 // loads in flight).  Helpers stop at a deadline or when the walker has finished, so the launch
 // ends whatever the scheduling.
@@ -46,11 +56,42 @@ namespace bl {
 namespace {
 
 typedef __attribute__((address_space(3))) void lds_void;
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ T bits_as(v4i r) {
+    static_assert(sizeof(T) == sizeof(v4i), "16-byte words only");
+    T x;
+    __builtin_memcpy(&x, &r, sizeof(T));
+    return x;
+}
+
+// LDS reads issued back to back and waited for in the same asm block (one round trip; the
+// outputs are early-clobber, so no read lands in a register a later read takes its address from)
+__device__ __forceinline__ void lds_read_stage(const void *f, const void *a, const void *b, const void *c, int &fl,
+                                               v4i &ra, v4i &rb, v4i &rc) {
+    __asm__ volatile(
+        "ds_read_b32 %0, %4\n\tds_read_b128 %1, %5\n\tds_read_b128 %2, %6\n\tds_read_b128 %3, %7\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(fl), "=&v"(ra), "=&v"(rb), "=&v"(rc)
+        : "v"((const lds_void *)f), "v"((const lds_void *)a), "v"((const lds_void *)b), "v"((const lds_void *)c)
+        : "memory");
+}
+
+__device__ __forceinline__ void lds_read4(const void *a, const void *b, const void *c, const void *d, v4i &ra, v4i &rb,
+                                          v4i &rc, v4i &rd) {
+    __asm__ volatile(
+        "ds_read_b128 %0, %4\n\tds_read_b128 %1, %5\n\tds_read_b128 %2, %6\n\tds_read_b128 %3, %7\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(ra), "=&v"(rb), "=&v"(rc), "=&v"(rd)
+        : "v"((const lds_void *)a), "v"((const lds_void *)b), "v"((const lds_void *)c), "v"((const lds_void *)d)
+        : "memory");
+}
 
 constexpr int LAG_T = 256;              // batch vertices; the walker's computing and loading threads
 constexpr int LAG_WG = 2 * LAG_T + 64;  // + one control wave
 constexpr int LAG_K = 4;                // stage slots: batch b + 3 is loaded during batch b
-constexpr int LAG_DMAX = 20;            // largest lag (ring slots of moved positions)
+constexpr int LAG_DMAX = 27;            // largest lag (ring slots of moved positions)
 constexpr int LAG_RS = 4;               // recent neighbours per vertex (ring offsets in an int4)
 constexpr int LAG_LEAD = 6;             // a batch's flag is read LAG_LEAD or LAG_LEAD + 1 batches ahead
 constexpr int LAG_DEC = 16;             // decision ring (batch % LAG_DEC)
@@ -76,9 +117,18 @@ struct LagArgs {
     long long ticks;             // a helper's longest wait
 };
 
+constexpr size_t LDS_MAX = 160 * 1024;  // LDS of one workgroup on gfx950
+
 size_t lag_lds_bytes(bool has_w, int D) {
     return sizeof(double2) * (size_t)D * LAG_T + (size_t)LAG_K * LAG_T * (16 + 16 + 16 + (has_w ? 16 : 0)) +
            sizeof(int) * (LAG_DEC + 4);
+}
+
+// The largest lag whose ring fits the LDS beside the stage (27 without weights, 23 with).
+int lag_dmax(bool has_w) {
+    int D = LAG_DMAX;
+    while (D > 0 && lag_lds_bytes(has_w, D) > LDS_MAX) --D;
+    return D;
 }
 
 // Publishes *flag = v to other workgroups after every store of this thread (and, through a
@@ -97,11 +147,26 @@ __device__ __forceinline__ void wbar() {
     __asm__ volatile("" ::: "memory");
 }
 
+// 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, relative error about 2^-25) refined by
+// two Newton steps y <- y (3/2 - (x/2) y^2), within a few ulp of the correctly rounded value.
+// Nine dependent operations where an IEEE square root and a division take about twenty-five.
+// x = 0 gives NaN (0 * inf), as the reference's 0 / 0 does.
+__device__ __forceinline__ double rsqrt2(double x) {
+    const double h = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 // One attraction term of vertex v toward pj, d * (Wk |d|) (algorithms.cpp:1055-1058), added to
-// (ax, ay).  The walker and the helpers both use it, so their pre_i agree bit for bit.
+// (ax, ay), with |d| = |d|^2 / sqrt(|d|^2) from rsqrt2 (coincident or nearly coincident vertices,
+// |d| < 2^-450, add nothing, where the reference adds at most 2^-900 W).  The walker and the
+// helpers both use it, so their pre_i agree bit for bit.
 __device__ __forceinline__ void attr_add(double2 pj, double2 v, double Wk, double &ax, double &ay) {
     const double dx = pj.x - v.x, dy = pj.y - v.y;
-    const double sc = Wk * sqrt(dx * dx + dy * dy);
+    const double d2 = dx * dx + dy * dy;
+    const double sc = d2 > 0x1p-900 ? Wk * (d2 * rsqrt2(d2)) : 0.0;
     ax = fma(dx, sc, ax);
     ay = fma(dy, sc, ay);
 }
@@ -175,11 +240,15 @@ __device__ void lag_helper(const LagArgs &a, int hid, int nhelp) {
             // (nearly) reached is formed by the walker
             if (ok > b - LAG_LEAD - 2) continue;
         }
-        if (t < bc) a.pre[i] = settled_sum<HAS_W>(a, e0, e1, c, w, v, r);
-        // every wave's stores have reached the L2 before thread 0's release writes it back
+        if (t < bc) {  // agent-scope stores: in memory once complete
+            const double2 f = settled_sum<HAS_W>(a, e0, e1, c, w, v, r);
+            __hip_atomic_store(&a.pre[i].x, f.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.pre[i].y, f.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // every wave's stores have completed before thread 0 flags the batch
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
-        if (t == 0) publish(a.ready + b, a.gen);
+        if (t == 0) __hip_atomic_store(a.ready + b, a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -202,37 +271,60 @@ __device__ void lag_walker(const LagArgs &a, unsigned char *raw) {
         for (int b = 0; b < nb; ++b) {
             const int s = (b % K) * T + t;
             const int b0 = b * B, i = b0 + t;
-            const bool flagged = dec[b % LAG_DEC] != 0;
+            // one LDS round trip: the batch's flag and this thread's whole stage
+            int fl;
+            v4i w0, w1, w2, w3 = {0, 0, 0, 0};
+            lds_read_stage(dec + b % LAG_DEC, sv + s, sp + s, so + s, fl, w0, w1, w2);
+            if (HAS_W) w3 = reinterpret_cast<const v4i *>(sw)[s];  // before the first use of w0..w2
+            const double2 v = bits_as<double2>(w0);
+            double2 p = bits_as<double2>(w1);
+            const int4 ro = bits_as<int4>(w2);
+            const float4 rw = bits_as<float4>(w3);
+            const bool flagged = __builtin_amdgcn_readfirstlane(fl) != 0;
             helped += flagged;
-            if (i < min(n, b0 + B)) {
-                const double2 v = sv[s];
-                double2 p = sp[s];
-                if (!flagged) {  // the helper was late (or the batch is one of the first): form pre_i
-                    const int e0 = a.erp[i], e1 = a.esp[i];
-                    int c[LAG_HS];
-                    float w[LAG_HS];
-                    settled_ids<HAS_W>(a, e0, e1, c, w);
-                    p = settled_sum<HAS_W>(a, e0, e1, c, w, v, p);
-                }
-                const int4 ro = so[s];
-                float4 rw = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (HAS_W) rw = sw[s];
-                const int roff[LAG_RS] = {ro.x, ro.y, ro.z, ro.w};
-                const float rwk[LAG_RS] = {rw.x, rw.y, rw.z, rw.w};
-#pragma unroll
-                for (int k = 0; k < LAG_RS; ++k)  // recent neighbours, from the ring, in CSR order
-                    if (roff[k] >= 0) attr_add(ring[roff[k]], v, HAS_W ? a.W * (double)rwk[k] : a.W, p.x, p.y);
-                const double mag = sqrt(p.x * p.x + p.y * p.y);  // x_i += (F/|F|) * STEP, :1062-1066
-                double2 q = v;
-                q.x = q.x + (p.x / mag) * a.step;
-                q.y = q.y + (p.y / mag) * a.step;
-                ring[(b % D) * T + t] = q;
-                a.pos[i] = q;
-                a.eterm[i] = p.x * p.x + p.y * p.y;
+            // the last batch's threads past n compute on stale stage words and store nothing
+            const bool live = i < min(n, b0 + B);
+            if (live && !flagged) {  // the helper was late (or the batch is one of the first): form pre_i
+                const int e0 = a.erp[i], e1 = a.esp[i];
+                int c[LAG_HS];
+                float w[LAG_HS];
+                settled_ids<HAS_W>(a, e0, e1, c, w);
+                p = settled_sum<HAS_W>(a, e0, e1, c, w, v, p);
             }
-            // the stores of batch b-3 are complete (two per batch, in order; the control wave
+            const int roff[LAG_RS] = {ro.x, ro.y, ro.z, ro.w};
+            const float rwk[LAG_RS] = {rw.x, rw.y, rw.z, rw.w};
+            // a second round trip: every ring slot this vertex may need (clamped offsets)
+            double2 rj[LAG_RS];
+            {
+                static_assert(LAG_RS == 4, "four ring reads");
+                v4i q0, q1, q2, q3;
+                lds_read4(ring + max(roff[0], 0), ring + max(roff[1], 0), ring + max(roff[2], 0),
+                          ring + max(roff[3], 0), q0, q1, q2, q3);
+                rj[0] = bits_as<double2>(q0);
+                rj[1] = bits_as<double2>(q1);
+                rj[2] = bits_as<double2>(q2);
+                rj[3] = bits_as<double2>(q3);
+            }
+#pragma unroll
+            for (int k = 0; k < LAG_RS; ++k)  // recent neighbours, from the ring, in CSR order
+                if (roff[k] >= 0) attr_add(rj[k], v, HAS_W ? a.W * (double)rwk[k] : a.W, p.x, p.y);
+            // x_i += (F/|F|) * STEP (:1062-1066), the unit vector F * rsqrt2(|F|^2)
+            const double m2 = p.x * p.x + p.y * p.y;
+            const double r = rsqrt2(m2) * a.step;
+            double2 q;
+            q.x = fma(p.x, r, v.x);
+            q.y = fma(p.y, r, v.y);
+            if (live) {
+                ring[(b % D) * T + t] = q;
+                // agent-scope stores are written through to memory, so the helpers (on other
+                // XCDs) read them once they have completed: no L2 write-back is needed to publish
+                __hip_atomic_store(&a.pos[i].x, q.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&a.pos[i].y, q.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                a.eterm[i] = m2;
+            }
+            // the stores of batch b-3 are complete (three per batch, in order; the control wave
             // publishes them after a barrier)
-            __asm__ volatile("s_waitcnt vmcnt(6)" ::: "memory");
+            __asm__ volatile("s_waitcnt vmcnt(9)" ::: "memory");
             wbar();  // end of batch b
         }
         if (t == 0) *shelp = helped;
@@ -297,14 +389,15 @@ __device__ void lag_walker(const LagArgs &a, unsigned char *raw) {
                 if (__ballot(good)) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 // batch xx is staged during batch xx - 3 >= b + 1, after this batch's barrier
                 if (l < 2 && xx < nb) dec[xx % LAG_DEC] = good ? 1 : 0;
-                // batches [0, b - 3): the computing waves completed batch b - 4's stores before
-                // the last barrier
-                if (l == 0 && b >= 4) publish(a.done, b - 3);
             }
             val = 0;
             const int y = b + LAG_LEAD + l;
             if (l < 2 && y < nb) val = __hip_atomic_load(a.ready + y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        // batches [0, b - 3): the computing waves completed batch b - 4's stores before the last
+        // barrier; they are agent-scope stores, in memory once complete, so a relaxed store
+        // publishes them (a release fence would write the L2 back: 0.6 us)
+        if (l == 0 && b >= 4) __hip_atomic_store(a.done, b - 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         wbar();  // end of batch b
     }
     wbar();
@@ -335,7 +428,7 @@ bool lag_walk_fits(bl_graph *g, int B, int D) {
     g->lag_ok = false;
     const int n = g->n;
     const int nb = (n + B - 1) / B;
-    if (!(B <= LAG_T && D >= 4 && D <= LAG_DMAX && nb >= 2 * D && nb >= LAG_LEAD + 4 && g->nnz > 0 &&
+    if (!(B <= LAG_T && D >= 4 && D <= lag_dmax(!g->h_w.empty()) && nb >= 2 * D && nb >= LAG_LEAD + 4 && g->nnz > 0 &&
           g->max_row <= LAG_MAXDEG && (int)g->h_rowptr.size() == n + 1 && (int64_t)g->h_colids.size() == g->nnz))
         return false;
     const bool hw = !g->h_w.empty();
@@ -388,6 +481,36 @@ bool lag_walk_fits(bl_graph *g, int B, int D) {
     return true;
 }
 
+// The lag the walk takes by default for batches of B vertices (0: the walk does not fit): the
+// largest that the LDS holds, at most half the batches, and at most the smallest distance, over
+// the vertices with five or more neighbours in earlier batches, from a vertex's batch to the
+// batch of its fifth-nearest such neighbour (so no vertex has more than LAG_RS = 4 recent
+// neighbours).  Cached per batch size.
+int lag_walk_choose(bl_graph *g, int B) {
+    if (g->lag_auto_batch == B) return g->lag_auto_d;
+    g->lag_auto_batch = B;
+    g->lag_auto_d = 0;
+    const int n = g->n;
+    const int nb = (n + B - 1) / B;
+    if (B > LAG_T || (int)g->h_rowptr.size() != n + 1 || g->max_row > LAG_MAXDEG) return 0;
+    int D = std::min(lag_dmax(!g->h_w.empty()), nb / 2);
+    static_assert(LAG_RS == 4, "the fifth-nearest gap bounds the lag");
+    for (int i = 0; i < n && D >= 4; ++i) {
+        const int bi = i / B;
+        int gap[LAG_RS + 1], ng = 0;  // the five smallest gaps bi - bj >= 1, ascending
+        for (int e = g->h_rowptr[i]; e < g->h_rowptr[i + 1]; ++e) {
+            const int k = bi - g->h_colids[e] / B;
+            if (k < 1 || (ng == LAG_RS + 1 && k >= gap[LAG_RS])) continue;
+            int m = ng < LAG_RS + 1 ? ng++ : LAG_RS;
+            while (m > 0 && gap[m - 1] > k) gap[m] = gap[m - 1], --m;
+            gap[m] = k;
+        }
+        if (ng == LAG_RS + 1) D = std::min(D, gap[LAG_RS]);  // recent: gaps 1 .. D - 1
+    }
+    g->lag_auto_d = D >= 4 ? D : 0;
+    return g->lag_auto_d;
+}
+
 // the count of batches the last lagged walk staged with the helpers' sums
 int *lag_walk_helped(bl_graph *g) { return g->lag_ready.as<int>() + (g->n + g->lag_batch - 1) / g->lag_batch + 33; }
 
@@ -431,7 +554,7 @@ int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, 
 }
 
 int lag_walk_prepare(bool has_w) {
-    const int lb = (int)lag_lds_bytes(has_w, LAG_DMAX);
+    const int lb = (int)lag_lds_bytes(has_w, lag_dmax(has_w));
     BL_HIP(hipFuncSetAttribute(has_w ? reinterpret_cast<const void *>(k_walk_lag<true>)
                                      : reinterpret_cast<const void *>(k_walk_lag<false>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, lb));

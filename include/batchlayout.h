@@ -112,8 +112,9 @@ typedef struct {
     int32_t far_model_iters;
     float far_judge_err[2];
     /* BL_WALK_LAG: batches of the last iteration whose sums over long-moved neighbours came from the
-       helper workgroups (the walker formed the others itself) */
+       helper workgroups (the walker formed the others itself), and the lag D it ran with */
     int32_t walk_helped;
+    int32_t walk_lag;
 } bl_stats;
 
 /* The batch walks of the Barnes-Hut family (bl_stats.walk); the library picks one per graph and
