@@ -42,8 +42,8 @@ void DevBuf::release() {
 }
 
 void BHWork::release() {
-    DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &hist, &spos, &frame, &isrep, &nodeid, &bnd_lo,
-                     &node_child, &node_depth, &node_cnt, &node_parent, &node_cxy, &node_rep, &scan_tmp,
+    DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &hist, &spos, &frame, &isrep, &nodeid,
+                     &node_child, &node_cnt, &node_lo, &scan_tmp,
                      &rep, &misc, &far_cells, &far_arrive, &judge, &gwalk};
     for (DevBuf *b : all) b->release();
     n_alloc = 0;

@@ -427,7 +427,8 @@ __global__ void k_tree_root(const uint64_t *__restrict__ keys, int n, const int 
 __global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ keys, int n,
                                                    const int *__restrict__ isrep, const int *__restrict__ nodeid,
                                                    BHNode *__restrict__ nodes, int *__restrict__ npts,
-                                                   int *__restrict__ misc, const int *__restrict__ status) {
+                                                   int *__restrict__ node_lo, int *__restrict__ misc,
+                                                   const int *__restrict__ status) {
     const int i = blockIdx.x * WG + threadIdx.x;
     if (*status) return;
     if (i >= n - 1 || !isrep[i]) return;
@@ -461,10 +462,13 @@ __global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ 
     nodes[v].nint = nint;
     nodes[v].cnt = 0;
     npts[v] = R - L + 1;
+    node_lo[v] = L;
     if (v == misc[1]) nodes[v].parent = -1;
 }
 
-// 6. centroids, bottom-up; the last child to finish computes its parent
+// 6. centroids, bottom-up; the last child to finish computes its parent.  WT: stored with
+// agent-scope stores, which the L2 writes through to memory
+template <bool WT = false>
 __device__ __forceinline__ void centroid_of(BHNode *nodes, const double2 *spos, int v) {
     const int4 ch = nodes[v].ch;
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
@@ -487,41 +491,65 @@ __device__ __forceinline__ void centroid_of(BHNode *nodes, const double2 *spos, 
         sx /= count;
         sy /= count;
     }
-    nodes[v].c = make_double2(sx, sy);
+    if (WT) {
+        __hip_atomic_store(&nodes[v].c.x, sx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&nodes[v].c.y, sy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        nodes[v].c = make_double2(sx, sy);
+    }
 }
 
 // Centroids bottom-up in two launches that keep the agent-scope handoffs to the top of the
 // tree.  A "small" node holds at most SMALL_CELL points.  k_centroid_small: one thread per
 // small subtree (a small node whose parent is not small) computes its nodes in post-order,
 // with no synchronisation.  k_centroid_top: the large nodes, from those without large
-// children upwards; the last large child to arrive at a parent (agent-scope acq_rel counter)
-// computes it; small children were done by the previous launch.  Every centroid is formed by
-// centroid_of from its children, so the order of the operations is the reference's.
+// children upwards; the last large child to arrive at a parent computes it; small children
+// were done by the previous launch.  Every centroid is formed by centroid_of from its
+// children, so the order of the operations is the reference's.
 constexpr int SMALL_CELL = 32;
-constexpr int SUB_STACK = 64;  // a small subtree has <= 31 internal nodes, each pushed twice
 
-__global__ __launch_bounds__(64) void k_centroid_small(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
-                                                       const int *__restrict__ npts, const int *__restrict__ misc) {
-    __shared__ int stk[64][SUB_STACK];
-    const int v = blockIdx.x * 64 + threadIdx.x;
-    if (misc[0] || v >= misc[2]) return;
-    const int par = nodes[v].parent;
-    if (npts[v] > SMALL_CELL || (par >= 0 && npts[par] <= SMALL_CELL)) return;
-    int *st = stk[threadIdx.x];
-    int sp = 0;
-    st[sp++] = v;
-    while (sp > 0) {
-        const int u = st[--sp];
-        if (u >= 0) {
-            st[sp++] = ~u;  // revisit after the children
-            const int4 ch = nodes[u].ch;
-            const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+// The first internal child of node u in quadrant order (-1: none).
+__device__ __forceinline__ int first_internal(const BHNode *nodes, int u) {
+    const int4 ch = nodes[u].ch;
+    return ch.x >= 0 ? ch.x : ch.y >= 0 ? ch.y : ch.z >= 0 ? ch.z : ch.w;  // ch.w: itself, or < 0
+}
+
+// The internal sibling after child u of node p in quadrant order (-1: none).
+__device__ __forceinline__ int next_internal(const BHNode *nodes, int p, int u) {
+    const int4 ch = nodes[p].ch;
+    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+    int nx = -1;
+    bool past = false;
 #pragma unroll
-            for (int q = 3; q >= 0; --q)
-                if (c[q] >= 0) st[sp++] = c[q];
-        } else {
-            centroid_of(nodes, spos, ~u);
+    for (int q = 0; q < 4; ++q) {
+        if (past && nx < 0 && c[q] >= 0) nx = c[q];
+        past = past || c[q] == u;
+    }
+    return nx;
+}
+
+// One thread per small subtree: its internal nodes in post-order (children in quadrant order)
+// without a stack, by the parent links (no LDS, so the mostly idle workgroups of the node-id
+// grid do not hold back occupancy).
+__global__ __launch_bounds__(WG) void k_centroid_small(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                       const int *__restrict__ npts, const int *__restrict__ misc) {
+    const int r = blockIdx.x * WG + threadIdx.x;
+    if (misc[0] || r >= misc[2]) return;
+    const int par = nodes[r].parent;
+    if (npts[r] > SMALL_CELL || (par >= 0 && npts[par] <= SMALL_CELL)) return;
+    int u = r;
+    for (int d; (d = first_internal(nodes, u)) >= 0;) u = d;
+    for (;;) {
+        centroid_of(nodes, spos, u);  // its internal children are done
+        if (u == r) break;
+        const int p = nodes[u].parent;
+        const int nx = next_internal(nodes, p, u);
+        if (nx < 0) {
+            u = p;
+            continue;
         }
+        u = nx;
+        for (int d; (d = first_internal(nodes, u)) >= 0;) u = d;
     }
 }
 
@@ -534,17 +562,24 @@ __device__ __forceinline__ int large_children(const BHNode *nodes, const int *np
     return k;
 }
 
+// The large nodes climb: a node's centroid is stored with agent-scope stores (written through
+// the XCD's L2 to memory), and once they have completed a relaxed counter at the parent counts
+// the arrival; the last child to arrive acquires (invalidating its own L2's stale lines) and
+// computes the parent from memory.  An acq_rel counter would write the whole L2 back at every
+// step.
 __global__ __launch_bounds__(WG) void k_centroid_top(BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
                                                      const int *__restrict__ npts, const int *__restrict__ misc) {
     int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
     if (npts[v] <= SMALL_CELL || large_children(nodes, npts, v) != 0) return;
     for (;;) {
-        centroid_of(nodes, spos, v);
+        centroid_of<true>(nodes, spos, v);
         const int u = nodes[v].parent;
         if (u < 0) break;
-        const int old = __hip_atomic_fetch_add(&nodes[u].cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const int old = __hip_atomic_fetch_add(&nodes[u].cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old != large_children(nodes, npts, u) - 1) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         v = u;
     }
 }
@@ -1820,15 +1855,17 @@ int ensure_bh(BHWork &w, int n) {
         w.scan_tmp.ensure(4 * (size_t)(256 * nblk + 1)) || w.spos.ensure(16 * N) ||
         w.frame.ensure(sizeof(double4) * 1024 + 64) || w.isrep.ensure(4 * std::max(N, (size_t)SCAN_SINGLE_MAX)) ||
         w.nodeid.ensure(4 * N) || w.node_child.ensure(sizeof(BHNode) * N) || w.rep.ensure(16 * N) ||
-        w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_cxy.ensure(16 * N) || w.node_cnt.ensure(4 * N))
+        w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_lo.ensure(4 * N) || w.node_cnt.ensure(4 * N))
         return BL_E_NOMEM;
     w.n_alloc = n;
     return BL_OK;
 }
 
 // Builds the tree of the positions d_pos (device, n points) into w; returns BL_OK and
-// leaves the status word in misc[0] (checked by the caller when it needs it).
-int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *d_xs, uint32_t *d_ys) {
+// leaves the status word in misc[0] (checked by the caller when it needs it).  centroids: the
+// unweighted centroids of algo 4's traversal (the far-field modes summarise cells themselves).
+int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *d_xs, uint32_t *d_ys,
+               bool centroids = true) {
     int rc = ensure_bh(w, n);
     if (rc) return rc;
     int *misc = w.misc.as<int>();
@@ -1868,10 +1905,13 @@ int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *
         const int nsb = (n - 1 + SCAN_TILE - 1) / SCAN_TILE;
         hipLaunchKernelGGL(k_tree_root, dim3(1), dim3(64), 0, s, ka, n, nodeid, bsum + nsb, misc);
         int *npts = w.node_cnt.as<int>();
-        hipLaunchKernelGGL(k_tree_nodes, dim3(g1), dim3(WG), 0, s, ka, n, isrep, nodeid, nodes, npts, misc, misc);
+        hipLaunchKernelGGL(k_tree_nodes, dim3(g1), dim3(WG), 0, s, ka, n, isrep, nodeid, nodes, npts,
+                           w.node_lo.as<int>(), misc, misc);
         // n-1 boundaries give at most n-1 nodes; threads beyond the node count (misc[2]) exit
-        hipLaunchKernelGGL(k_centroid_small, dim3((n + 63) / 64), dim3(64), 0, s, nodes, spos, npts, misc);
-        hipLaunchKernelGGL(k_centroid_top, dim3(g1), dim3(WG), 0, s, nodes, spos, npts, misc);
+        if (centroids) {
+            hipLaunchKernelGGL(k_centroid_small, dim3(g1), dim3(WG), 0, s, nodes, spos, npts, misc);
+            hipLaunchKernelGGL(k_centroid_top, dim3(g1), dim3(WG), 0, s, nodes, spos, npts, misc);
+        }
     } else {
         const int neg = -1;
         BL_HIP(hipMemcpyAsync(misc + 1, &neg, sizeof(int), hipMemcpyHostToDevice, s));
@@ -1993,7 +2033,7 @@ int launch_walk_grid(bl_graph *g, double2 *pos, const bl_params *p, double step,
 }  // namespace
 
 int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s) {
-    return build_tree(w, d_pos, n, s, nullptr, nullptr);
+    return build_tree(w, d_pos, n, s, nullptr, nullptr, false);
 }
 
 int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iters, double theta, bl_stats *st,
@@ -2051,6 +2091,7 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     int judge_every = 10;
     if (const char *je = getenv("BL_FAR_JUDGE")) judge_every = atoi(je);
     const bool judged = far_mode == FAR_LEARNED && judge_every > 0 && n >= 2;
+    int verdict = 1;  // the judge's last verdict (1: the model)
     if (judged) {
         rc = far_judge_reset(g->bh, s);
         if (rc) return rc;
@@ -2060,7 +2101,8 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     for (int it = 0; it < iters; ++it) {
         if (far_mode != FAR_EXACT) {
             prof.begin(3);
-            rc = build_tree(g->bh, pos, n, s, nullptr, nullptr);  // tree of the iteration-start positions
+            // tree of the iteration-start positions (centroids for algo 4's traversal only)
+            rc = build_tree(g->bh, pos, n, s, nullptr, nullptr, far_mode == 0);
             prof.end();
             if (rc) return rc;
         }
@@ -2074,12 +2116,15 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         }
         if (judged && it % judge_every == 0) {
             prof.begin(6);
-            rc = far_judge(g->bh, pos, n, theta, model, it, s);
+            rc = far_judge(g->bh, pos, n, theta, model, it, s, &verdict);
             prof.end();
             if (rc) return rc;
         }
+        // the learned mode takes the analytic second-order kernel while the model's last verdict
+        // was against it
+        const int mode_now = (far_mode == FAR_LEARNED && judged && !verdict) ? (int)FAR_QUAD : far_mode;
         prof.begin(0);
-        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, far_mode, model, s, ncells, judged)
+        rc = far_mode ? far_repulsion(g->bh, pos, n, theta, mode_now, model, s, ncells, judged)
                       : launch_rep(g->bh, n, theta, p->precision, s);
         prof.end();
         if (rc) return rc;

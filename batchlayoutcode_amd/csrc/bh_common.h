@@ -22,7 +22,9 @@ constexpr int ST_DUPKEY = 1;
 
 // Builds the tree of the n points d_pos into w: w.keys_a / w.idx_a sorted keys and vertex
 // ids, w.spos positions in Morton order, w.node_child nodes, w.frame = {minX, minY, MAXMIN},
-// misc[1] = root node (-1 for a single point), misc[2] = node count.
+// misc[1] = root node (-1 for a single point), misc[2] = node count.  The nodes' unweighted
+// centroids (BHNode.c, algo 4's traversal) are not formed: the far field summarises the cells
+// itself (far_cells_of).
 int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 
 // Repulsion of every vertex for the physically consistent layouts (farfield.hip): into
@@ -49,9 +51,11 @@ int far_repulsion(BHWork &w, const double2 *d_pos, int n, double theta, int mode
                   hipStream_t s, unsigned long long *ncells, bool judged);
 // The learned field's judge: the exact repulsion at a sample of the positions d_pos (of the tree
 // in w), against the learned and the analytic second-order fields there; sets the verdict on
-// the device.  far_judge_reset starts a run (model on, counters cleared); far_judge_read queues
-// the copies of the number of traversals that used the model and the last medians.
-int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float *model, int it, hipStream_t s);
+// the device and returns it in *verdict (1: the model; waits for the stream).  far_judge_reset
+// starts a run (model on, counters cleared); far_judge_read queues the copies of the number of
+// traversals that used the model and the last medians.
+int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float *model, int it, hipStream_t s,
+              int *verdict);
 int far_judge_reset(BHWork &w, hipStream_t s);
 int far_judge_read(BHWork &w, int *model_iters, float *err, hipStream_t s);
 

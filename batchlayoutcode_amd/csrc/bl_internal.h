@@ -37,8 +37,8 @@ struct BHWork {
     DevBuf spos;                             // positions in Morton order (double2)
     DevBuf frame;                            // minx, miny, maxmin (+ block partials)
     DevBuf isrep, nodeid;                    // boundary representative flags and node ids
-    DevBuf bnd_lo;                           // node range start per boundary
-    DevBuf node_child, node_depth, node_cnt, node_parent, node_cxy, node_rep;
+    DevBuf node_child;                       // nodes (BHNode)
+    DevBuf node_cnt, node_lo;                // points of each node: Morton ranks [lo, lo + cnt)
     DevBuf scan_tmp;
     DevBuf rep;                              // repulsion result per vertex (double2)
     DevBuf misc;                             // counters / status words

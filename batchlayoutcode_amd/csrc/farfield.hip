@@ -46,7 +46,7 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 
 // Combines the non-empty quadrants of node v (children first): mean by counts, second
 // moments by the parallel-axis rule, in quadrant order.
-__device__ void far_summary(const BHNode *nodes, const double2 *spos, FarCell *cells, int v) {
+__device__ FarCell far_summary(const BHNode *nodes, const double2 *spos, const FarCell *cells, int v) {
     const int4 ch = nodes[v].ch;
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
     double m = 0.0, sx = 0.0, sy = 0.0;
@@ -92,39 +92,63 @@ __device__ void far_summary(const BHNode *nodes, const double2 *spos, FarCell *c
     out.myy = d;
     out.m = (int)m;
     out.lo = lo;
-    cells[v] = out;
+    return out;
 }
 
-// The summaries bottom-up in two launches (as the centroids of bh.hip): one thread per
-// small subtree (at most FAR_SMALL points) in post-order, then the large nodes, where the
-// last large child to arrive at a parent (agent-scope acq_rel counter) computes it.
-constexpr int FAR_SMALL = 32;
-constexpr int FAR_STACK = 64;
+// Stores cell v with agent-scope stores, which the XCD's L2 writes through to memory.
+__device__ __forceinline__ void store_cell_through(FarCell *cells, int v, const FarCell &c) {
+    static_assert(sizeof(FarCell) == 48, "five doubles, then m and lo");
+    double *d = reinterpret_cast<double *>(cells + v);
+    __hip_atomic_store(d + 0, c.cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 1, c.cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 2, c.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 3, c.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d + 4, c.myy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long ml = (unsigned long long)(unsigned)c.m | ((unsigned long long)(unsigned)c.lo << 32);
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(d + 5), ml, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-__global__ __launch_bounds__(64) void k_far_small(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
-                                                  const int *__restrict__ npts, const int *__restrict__ misc,
-                                                  FarCell *__restrict__ cells) {
-    __shared__ int stk[64][FAR_STACK];
-    const int v = blockIdx.x * 64 + threadIdx.x;
+// The summaries in two launches.  A node of at most FAR_SMALL points is summarised straight
+// from its points, Morton ranks [lo, lo + m), in two passes (the mean, then the central second
+// moments): one thread per node, no dependence between nodes.  The larger nodes climb, from
+// those without large children: a node's summary is stored with agent-scope stores (written
+// through to memory) and, once they have completed, a relaxed counter at the parent counts the
+// arrival; the last child to arrive acquires (invalidating its own L2's stale lines) and forms
+// the parent from its children by far_summary.  An acq_rel counter would write the whole L2
+// back at every step.
+constexpr int FAR_SMALL = 32;
+
+__global__ __launch_bounds__(WG) void k_far_direct(const double2 *__restrict__ spos, const int *__restrict__ npts,
+                                                   const int *__restrict__ node_lo, const int *__restrict__ misc,
+                                                   FarCell *__restrict__ cells) {
+    const int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
-    const int par = nodes[v].parent;
-    if (npts[v] > FAR_SMALL || (par >= 0 && npts[par] <= FAR_SMALL)) return;
-    int *st = stk[threadIdx.x];
-    int sp = 0;
-    st[sp++] = v;
-    while (sp > 0) {
-        const int u = st[--sp];
-        if (u >= 0) {
-            st[sp++] = ~u;
-            const int4 ch = nodes[u].ch;
-            const int c[4] = {ch.x, ch.y, ch.z, ch.w};
-#pragma unroll
-            for (int q = 3; q >= 0; --q)
-                if (c[q] >= 0) st[sp++] = c[q];
-        } else {
-            far_summary(nodes, spos, cells, ~u);
-        }
+    const int m = npts[v];
+    if (m > FAR_SMALL) return;
+    const int lo = node_lo[v];
+    double sx = 0.0, sy = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double2 p = spos[lo + k];
+        sx += p.x;
+        sy += p.y;
     }
+    FarCell out;
+    out.cx = sx / m;
+    out.cy = sy / m;
+    double a = 0.0, b = 0.0, d = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double2 p = spos[lo + k];
+        const double ux = p.x - out.cx, uy = p.y - out.cy;
+        a += ux * ux;
+        b += ux * uy;
+        d += uy * uy;
+    }
+    out.mxx = a;
+    out.mxy = b;
+    out.myy = d;
+    out.m = m;
+    out.lo = lo;
+    cells[v] = out;
 }
 
 __device__ __forceinline__ int far_large_children(const BHNode *nodes, const int *npts, int v) {
@@ -143,11 +167,13 @@ __global__ __launch_bounds__(WG) void k_far_top(const BHNode *__restrict__ nodes
     if (misc[0] || v >= misc[2]) return;
     if (npts[v] <= FAR_SMALL || far_large_children(nodes, npts, v) != 0) return;
     for (;;) {
-        far_summary(nodes, spos, cells, v);
+        store_cell_through(cells, v, far_summary(nodes, spos, cells, v));
         const int u = nodes[v].parent;
         if (u < 0) break;
-        const int old = __hip_atomic_fetch_add(&arrive[u], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const int old = __hip_atomic_fetch_add(&arrive[u], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old != far_large_children(nodes, npts, u) - 1) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         v = u;
     }
 }
@@ -617,8 +643,8 @@ int far_cells_of(BHWork &w, int n, hipStream_t s) {
     if (rc) return rc;
     if (n >= 2) {
         BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
-        hipLaunchKernelGGL(k_far_small, dim3((n + 63) / 64), dim3(64), 0, s, w.node_child.as<BHNode>(),
-                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>());
+        hipLaunchKernelGGL(k_far_direct, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.spos.as<double2>(),
+                           w.node_cnt.as<int>(), w.node_lo.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>());
         hipLaunchKernelGGL(k_far_top, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
                            w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>(),
                            w.far_arrive.as<int>());
@@ -627,7 +653,8 @@ int far_cells_of(BHWork &w, int n, hipStream_t s) {
     return BL_OK;
 }
 
-int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float *model, int it, hipStream_t s) {
+int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float *model, int it, hipStream_t s,
+              int *verdict) {
     if (w.judge.ensure(sizeof(JudgeBuf))) return BL_E_NOMEM;
     JudgeBuf *J = w.judge.as<JudgeBuf>();
     const int S = std::min(JUDGE_S, n);
@@ -642,6 +669,10 @@ int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float 
     if (rc) return rc;
     hipLaunchKernelGGL(k_judge_decide, dim3(1), dim3(JUDGE_S), 0, s, S, J);
     BL_HIP(hipGetLastError());
+    // the host launches the traversal the verdict names: the analytic kernel alone holds fewer
+    // registers than the learned one with its analytic branch (one wait per judged iteration)
+    BL_HIP(hipMemcpyAsync(verdict, &J->use_model, sizeof(int), hipMemcpyDeviceToHost, s));
+    BL_HIP(hipStreamSynchronize(s));
     return BL_OK;
 }
 

@@ -10,6 +10,8 @@
   rmat20-algo8     the same graph, algo 8, -batch 1024
   delaunay2m-far   Delaunay edges of 2M points minus 20 disks (seed 3), LCC; physically consistent
                    Barnes-Hut: monopole, quadrupole and learned far field (model trained first), -bht 1.0
+  delaunay2m-algo40  the same graph, monopole Barnes-Hut alone at -bht 0.4 (no model: a profile of the
+                   plain path with nothing else in the process)
   batched4096      4,096 small graphs (64..8192 vertices, seed 4), algo 2, -batch 128, one launch
 
 Initial coordinates are uniform in [0,1)^2 (seed 0).  Prints one JSON line per run.
@@ -24,7 +26,7 @@ import numpy as np
 from . import api, graphs
 
 # the library's kernel classes (bl_stats.kernel_ms)
-CLASSES = ("pairs_or_traversal", "move_or_walk", "energy", "tree", "cell_summaries", "walk_prepass")
+CLASSES = ("pairs_or_traversal", "move_or_walk", "energy", "tree", "cell_summaries", "walk_prepass", "judge")
 
 
 def _run(tag, rp, ci, w, iters, profile, **kw):
@@ -37,7 +39,7 @@ def _run(tag, rp, ci, w, iters, profile, **kw):
                vertex_updates_per_s=n * iters / st.seconds, energy=st.energy, params=kw)
     if profile:
         rec["kernel_ms_per_iter"] = dict(zip(CLASSES, [m / iters for m in st.kernel_ms]))
-        rec["walk"] = st.walk
+        rec.update(walk=st.walk, walk_lag=st.walk_lag, walk_batches_helped=st.walk_helped, far_cells=st.far_cells)
     return rec
 
 
@@ -63,6 +65,10 @@ def main(argv=None):
         rp, ci, wc = graphs.csr_from_edges(n, e, w)
         algo = 4 if a.name.endswith("4") else 8
         out.append(_run(a.name, rp, ci, wc, a.iters, a.profile, algo=algo, batch=1024, bht=1.2,
+                        precision=a.precision))
+    elif a.name == "delaunay2m-algo40":
+        rp, ci = graphs.csr_from_edges(*graphs.delaunay_holes())
+        out.append(_run(a.name, rp, ci, None, a.iters, a.profile, algo=api.ALGO_FAR_MONO, batch=256, bht=0.4,
                         precision=a.precision))
     elif a.name == "delaunay2m-far":
         from . import farmodel

@@ -186,7 +186,7 @@ def c4_leg(args, torch, api, graphs):
                 tm.run(algo=algo, iters=1, batch=B, bht=th)
             secs, st = tm.run(algo=algo, iters=K, batch=B, bht=th)
             _, sp = tm.run(algo=algo, iters=K, batch=B, bht=th, profile=1)
-            cls = ("traversal", "walk", "energy", "tree", "cell_summaries", "walk_prepass")
+            cls = ("traversal", "walk", "energy", "tree", "cell_summaries", "walk_prepass", "judge")
             per_iter = {c: sp.kernel_ms[k] / K for k, c in enumerate(cls)}
             return secs, st, sp, per_iter
 
@@ -199,13 +199,19 @@ def c4_leg(args, torch, api, graphs):
                 continue
             secs, st, sp, per_iter = timed(algo, q["theta"])
             res[name] = dict(theta=q["theta"], value=n * K / secs, ms_per_step=secs * 1e3 / K, energy=st.energy,
-                             kernel_ms_per_iter=per_iter, far_cells_per_iter=sp.far_cells, walk=sp.walk)
+                             kernel_ms_per_iter=per_iter, far_cells_per_iter=sp.far_cells, walk=sp.walk,
+                             walk_lag=sp.walk_lag, walk_batches_helped=sp.walk_helped,
+                             model_iterations=sp.far_model_iters if algo == api.ALGO_FAR_LEARNED else None)
             log(f"{name} timed at theta {q['theta']}: {K} iterations in {secs:.3f} s = {n * K / secs:.3e} v-u/s; "
                 f"per iteration {json.dumps({k: round(v, 3) for k, v in per_iter.items()})} ms; "
                 f"cells {sp.far_cells}")
             if name == "learned":
                 res["_profile"] = (sp, per_iter)
     return n, rp, ci, init, out, res
+
+
+WALK_KERNELS = {1: "k_attract_move", 2: "k_walk_pf", 3: "k_walk_split", 4: "k_late_attract + k_walk_row",
+                5: "k_walk_grid", 6: "k_late_attract + k_walk_lag"}
 
 
 def c4_roofline(n, nnz, lp):
@@ -224,7 +230,7 @@ def c4_roofline(n, nnz, lp):
     # the batch walk: attraction over every CSR entry with live positions, then the move
     bytes_per_iter = n * (4 + 16 + 16 + 16 + 8) + nnz * (4 + 16)
     achieved = bytes_per_iter / (walk_ms / 1e3) / 1e9
-    return dict(bound="hbm", kernel="k_late_attract + k_walk_row", achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
+    return dict(bound="hbm", kernel=WALK_KERNELS.get(sp.walk, "walk"), achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
                 frac=achieved / PEAK_HBM_GBS, traffic=None, mean_launch_us=walk_ms * 1e3,
                 launches=sp.kernel_launches[1], bytes_per_launch=bytes_per_iter,
                 note="algorithmic bytes of one iteration's attraction and move (4 B rowptr + 16 B own position + "
