@@ -559,12 +559,17 @@ struct HostRun {
 // the traversals of the iteration then use the model only while its median relative force
 // error on the sample is at most the analytic term's.
 constexpr int JUDGE_S = 256;   // sample vertices
-constexpr int JUDGE_P = 256;   // j chunks of the exact sum
+constexpr int JUDGE_P = 1024;  // j chunks of the exact sum (four workgroups per CU)
 
 // layout of BHWork::judge
 struct JudgeBuf {
     double2 q[JUDGE_S];            // sample positions
-    int qid[JUDGE_S];              // 0..S-1 (output slots of the traversals)
+    // the traversals' queries: sample vertex k in all 64 lanes of wave k (output slot k).  The
+    // sample is spread over the layout, so a wave of 64 different sample vertices would walk
+    // the union of 64 unrelated paths one dependent node load at a time (about 5 ms per
+    // traversal on 1.7M vertices); one vertex per wave walks one path, the waves in parallel.
+    double2 qw[JUDGE_S * 64];
+    int qwid[JUDGE_S * 64];
     double2 part[JUDGE_S * JUDGE_P];
     double2 field[2 * JUDGE_S];    // learned, then analytic, at the sample
     int use_model;                 // the verdict read by the traversals
@@ -577,8 +582,12 @@ __global__ __launch_bounds__(JUDGE_S) void k_judge_sample(const double2 *__restr
     const int k = threadIdx.x;
     if (k >= S) return;
     const long long id = (long long)k * n / S + offset;
-    J->q[k] = pos[id < n ? id : n - 1];
-    J->qid[k] = k;
+    const double2 q = pos[id < n ? id : n - 1];
+    J->q[k] = q;
+    for (int l = 0; l < 64; ++l) {
+        J->qw[64 * k + l] = q;
+        J->qwid[64 * k + l] = k;
+    }
 }
 
 // part[k * P + c] = sum over points of chunk c of -(x_j - q_k)/|x_j - q_k|^2 (FP64-grade terms)
@@ -663,9 +672,9 @@ int far_judge(BHWork &w, const double2 *d_pos, int n, double theta, const float 
     hipLaunchKernelGGL(k_judge_sample, dim3(1), dim3(JUDGE_S), 0, s, d_pos, n, S, offset, J);
     hipLaunchKernelGGL(k_judge_exact, dim3(JUDGE_P), dim3(JUDGE_S), 0, s, d_pos, n, S, J);
     BL_HIP(hipGetLastError());
-    int rc = launch_trav(w, J->q, J->qid, S, theta, FAR_LEARNED, model, J->field, s);
+    int rc = launch_trav(w, J->qw, J->qwid, 64 * S, theta, FAR_LEARNED, model, J->field, s);
     if (rc) return rc;
-    rc = launch_trav(w, J->q, J->qid, S, theta, FAR_QUAD, model, J->field + JUDGE_S, s);
+    rc = launch_trav(w, J->qw, J->qwid, 64 * S, theta, FAR_QUAD, model, J->field + JUDGE_S, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_judge_decide, dim3(1), dim3(JUDGE_S), 0, s, S, J);
     BL_HIP(hipGetLastError());
