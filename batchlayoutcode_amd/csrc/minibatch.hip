@@ -20,6 +20,9 @@
 //     chunk workgroups write each batch vertex's chunk partials; its correction waves take
 //     the neighbour corrections in pieces of at most CORR_PIECE entries (a hub's
 //     correction is spread over many waves); k_move32 adds them in a fixed order and moves.
+//   * With BL_GRAPH=1, one iteration's launches (every batch step, the energy, STEP *= decay
+//     on the device) are captured once as a hipGraph and replayed, one host launch per
+//     iteration (measured no faster on this GPU, so not the default).
 //
 // FP64 mode (reference-grade): the reference's own operations, in its order.  The
 // j range is cut into the reference's 512-wide blocks (blocky, :717); each block's
@@ -28,6 +31,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 #include <mutex>
@@ -172,7 +176,7 @@ __global__ __launch_bounds__(WG) void k_pairs64(const double2 *__restrict__ pos,
 // pieces (FP64, fixed order: strided per thread, then a fixed tree).
 __global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part, int P, int G,
                                                const int *__restrict__ cpre, double2 *__restrict__ pos, int b0,
-                                               int bc, double step, double *__restrict__ eterm) {
+                                               int bc, const double *__restrict__ stepp, double *__restrict__ eterm) {
     __shared__ double2 red[WG / 64];
     const int slot = (int)blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -203,14 +207,15 @@ __global__ __launch_bounds__(WG) void k_move32(const double2 *__restrict__ part,
     if (tid == 0) {
         const double fx = (red[0].x + red[1].x) + (red[2].x + red[3].x);
         const double fy = (red[0].y + red[1].y) + (red[2].y + red[3].y);
-        pos[i] = moved(x0, fx, fy, step);
+        pos[i] = moved(x0, fx, fy, *stepp);
         eterm[i] = fx * fx + fy * fy;
     }
 }
 
 // FP64 mode: prevCoordinates[i] += f per j block in block order (:771), then the move.
 __global__ __launch_bounds__(WG) void k_move64(const double2 *__restrict__ part, int P, double2 *__restrict__ pos,
-                                               int b0, int bc, double step, double *__restrict__ eterm) {
+                                               int b0, int bc, const double *__restrict__ stepp,
+                                               double *__restrict__ eterm) {
     const int slot = (int)blockIdx.x * WG + threadIdx.x;
     if (slot >= bc) return;
     const double2 *row = part + (size_t)slot * P;
@@ -219,7 +224,17 @@ __global__ __launch_bounds__(WG) void k_move64(const double2 *__restrict__ part,
         sx += row[p].x;
         sy += row[p].y;
     }
-    move_one(pos, b0 + slot, sx, sy, step, eterm);
+    move_one(pos, b0 + slot, sx, sy, *stepp, eterm);
+}
+
+// STEP lives on the device so that one iteration's launches are the same every iteration (and
+// can be replayed as a captured graph): set once, then multiplied after each iteration with the
+// host's FP64 operation.
+__global__ void k_step_set(double *stepp, double v) {
+    if (threadIdx.x == 0) *stepp = v;
+}
+__global__ void k_step_decay(double *stepp, double decay) {
+    if (threadIdx.x == 0) *stepp = *stepp * decay;  // :781 (0.999), :934 / :1007 (0.99)
 }
 
 }  // namespace
@@ -369,34 +384,28 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
     Prof prof;
     prof.on = p->profile != 0;
     prof.s = s;
-    double STEP = step0;
-    double E = 0.0, E0 = 1.79769313486231570815e+308;
-    int loop = 0;
-    for (;;) {
-        if (converged) {
-            if (!(fabs(E0 - E) / E0 > p->engt)) break;  // algorithms.cpp:818
-        } else if (loop >= iters) {
-            break;
-        }
-        E0 = E;
-        {
-            for (int b = 0; b < nb; ++b) {
-                const int b0 = b * B;
-                const int bc = (n - b0) < B ? (n - b0) : B;
-                if (fp64) {
-                    prof.begin(0);
-                    hipLaunchKernelGGL(k_pairs64, dim3(G, (bc + WG - 1) / WG), dim3(WG), 0, s, pos, n, b0, bc, part,
-                                       P, erp, eci, ew, p->weight, law);
-                    prof.end();
-                    prof.begin(1);
-                    hipLaunchKernelGGL(k_move64, dim3((bc + WG - 1) / WG), dim3(WG), 0, s, part, P, pos, b0, bc,
-                                       STEP, eterm);
-                    prof.end();
-                } else {
-                    const int ipw = WG / JS;
-                    const int pc0 = g->h_cpre[b0], npc = g->h_cpre[b0 + bc] - pc0;
-                    const dim3 grid(G + (npc + 3) / 4, (bc + ipw - 1) / ipw);
-                    const int *cpre = g->cpre.as<int>(), *cpv = g->cpv.as<int>();
+    double *d_step = energy + 1;  // energy[0] is the sum, energy[4 ..] its shares
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, s, d_step, step0);
+    // One iteration's launches: every batch step (pairs, then the move), the energy, STEP *= decay.
+    auto emit_iteration = [&]() -> int {
+        for (int b = 0; b < nb; ++b) {
+            const int b0 = b * B;
+            const int bc = (n - b0) < B ? (n - b0) : B;
+            if (fp64) {
+                prof.begin(0);
+                hipLaunchKernelGGL(k_pairs64, dim3(G, (bc + WG - 1) / WG), dim3(WG), 0, s, pos, n, b0, bc, part, P,
+                                   erp, eci, ew, p->weight, law);
+                prof.end();
+                prof.begin(1);
+                hipLaunchKernelGGL(k_move64, dim3((bc + WG - 1) / WG), dim3(WG), 0, s, part, P, pos, b0, bc, d_step,
+                                   eterm);
+                prof.end();
+            } else {
+                const int ipw = WG / JS;
+                const int pc0 = g->h_cpre[b0], npc = g->h_cpre[b0 + bc] - pc0;
+                const dim3 grid(G + (npc + 3) / 4, (bc + ipw - 1) / ipw);
+                const int *cpre = g->cpre.as<int>(), *cpv = g->cpv.as<int>();
+                if (prof.on) {  // the dispatch records its own start and end
                     hipEvent_t e0, e1;
                     prof.ext(0, &e0, &e1);
                     if (p->precision == BL_PREC_FP32)
@@ -407,16 +416,67 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
                                               chunk, G, part, P, erp, eci, ew, p->weight, law, cpre, cpv, pc0, npc);
                     prof.ext(1, &e0, &e1);
                     hipExtLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, e0, e1, 0, part, P, G, cpre, pos, b0, bc,
-                                          STEP, eterm);
+                                          d_step, eterm);
+                } else {
+                    if (p->precision == BL_PREC_FP32)
+                        hipLaunchKernelGGL((k_pairs32<4, false>), grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part,
+                                           P, erp, eci, ew, p->weight, law, cpre, cpv, pc0, npc);
+                    else
+                        hipLaunchKernelGGL((k_pairs32<4, true>), grid, dim3(WG), 0, s, pos, n, b0, bc, chunk, G, part,
+                                           P, erp, eci, ew, p->weight, law, cpre, cpv, pc0, npc);
+                    hipLaunchKernelGGL(k_move32, dim3(bc), dim3(WG), 0, s, part, P, G, cpre, pos, b0, bc, d_step,
+                                       eterm);
                 }
             }
         }
         prof.begin(2);
-        int rc = launch_energy(eterm, n, energy, p->precision, s);
+        const int rc = launch_energy(eterm, n, energy, p->precision, s);
         prof.end();
-        if (rc) return rc;
-        prof.flush();
-        STEP = STEP * decay;  // :781 (0.999), :934 / :1007 (0.99)
+        hipLaunchKernelGGL(k_step_decay, dim3(1), dim3(64), 0, s, d_step, decay);
+        return rc;
+    };
+    // BL_GRAPH=1 (and no per-launch timing): one iteration is captured once as a graph and
+    // replayed, one host launch per iteration instead of two per batch step.  Not the default:
+    // measured on MI355X (ROCm 7) the replay was no faster (64x64 grid 96 vs 95 us per iteration,
+    // 32x32 grid 32 vs 28 us, 250k vertices 28.6 vs 28.0 ms): the GPU's own dispatch of
+    // dependent small kernels, not the host's launch calls, sets the pace.  The static-workspace
+    // path creates no graph (instantiation allocates).
+    struct Exec {
+        hipGraphExec_t e = nullptr;
+        ~Exec() {
+            if (e) (void)hipGraphExecDestroy(e);
+        }
+    } ex;
+    const char *genv = getenv("BL_GRAPH");
+    const bool want_graph = !prof.on && !static_ws && (converged || iters >= 2) && genv && !strcmp(genv, "1");
+    if (want_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int rc = emit_iteration();
+        hipGraph_t gr = nullptr;
+        const hipError_t ee = hipStreamEndCapture(s, &gr);
+        if (rc == BL_OK && ee == hipSuccess && gr) {
+            if (hipGraphInstantiate(&ex.e, gr, nullptr, nullptr, 0) != hipSuccess) ex.e = nullptr;
+        }
+        if (gr) (void)hipGraphDestroy(gr);
+        (void)hipGetLastError();  // a failed capture leaves the launches to the loop below
+    }
+    double STEP = step0;
+    double E = 0.0, E0 = 1.79769313486231570815e+308;
+    int loop = 0;
+    for (;;) {
+        if (converged) {
+            if (!(fabs(E0 - E) / E0 > p->engt)) break;  // algorithms.cpp:818
+        } else if (loop >= iters) {
+            break;
+        }
+        E0 = E;
+        if (ex.e) {
+            BL_HIP(hipGraphLaunch(ex.e, s));
+        } else {
+            const int rc = emit_iteration();
+            if (rc) return rc;
+            prof.flush();
+        }
+        STEP = STEP * decay;  // the host's copy (the device multiplies its own the same way)
         ++loop;
         if (converged) {
             BL_HIP(hipMemcpyAsync(&E, energy, sizeof(double), hipMemcpyDeviceToHost, s));

@@ -129,6 +129,33 @@ def test_fp32_run_is_deterministic(bl):
     assert np.array_equal(a, b) and ea.energy == eb.energy
 
 
+@pytest.mark.parametrize("algo,prec,conv", [(2, 0, 0), (8, 2, 0), (3, 0, 0), (2, 1, 0), (2, 0, 1)])
+def test_graph_replay_matches_launches(bl, monkeypatch, algo, prec, conv):
+    """One iteration captured as a hipGraph and replayed (BL_GRAPH=1) against the same launches
+    made one by one (the default): bit for bit, over runs of 1..5 replays made one after another with the
+    host reading the results between them (the converged mode also reads the energy back after
+    every replay), and with a run of the other kind in between."""
+    rp, ci = bl.mtx_read(golden_input("jagmesh1.mtx"))
+    x0 = np.random.default_rng(algo + 10 * prec).random((len(rp) - 1, 2))
+    kw = dict(algo=algo, batch=64, precision=prec)
+    if conv:
+        kw.update(converged=1, engt=0.02)
+    print(f"[case] graph replay algo {algo} precision {prec} converged {conv}", flush=True)
+    with bl.Graph(rp, ci) as g:
+        for iters in ((0,) if conv else (1, 2, 3, 5)):
+            out = {}
+            for mode in ("0", "1", "0"):
+                monkeypatch.setenv("BL_GRAPH", mode)
+                xy, st = g.run(x0, iters=iters, **kw)
+                if mode in out:
+                    assert np.array_equal(xy, out[mode][0]) and st.energy == out[mode][1].energy
+                out[mode] = (xy, st)
+            (a, sa), (b, sb) = out["0"], out["1"]
+            assert sa.iters_done == sb.iters_done and sa.iters_done >= max(iters, 1)
+            assert np.array_equal(a, b), (iters, np.abs(a - b).max())
+            assert sa.energy == sb.energy and sa.step_final == sb.step_final
+
+
 def _random_graph(rng, n, m, empty_rows=(), dup=False):
     e = rng.integers(0, n, size=(m, 2))
     e = e[e[:, 0] != e[:, 1]]
