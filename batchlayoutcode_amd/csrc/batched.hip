@@ -9,15 +9,23 @@
 // broadcast) and are read from global memory otherwise.  A graph never shares data with
 // another workgroup, so the only cross-workgroup traffic is the queue counter.
 //
-// Inside a batch step the IPW = next power of two >= min(B, n) batch slots are split over
-// JS = 1024 / IPW j-subgroups; the subgroup partials are added in subgroup order, then the
-// neighbour correction, so results do not depend on timing.
+// Inside a batch step IPW batch slots (a power of two, 64..1024, per graph) are served at once
+// by JS = 1024 / IPW j-subgroups, in rounds of IPW slots.  The j range is cut into NCH = 16
+// fixed chunks whose partials are combined by one fixed pairwise tree, ((c0 + c1) + (c2 + c3))
+// + ..., whatever the shape: a subgroup sums its 16 / JS consecutive chunks as the subtree they
+// form, and the subgroups' subtrees are combined as the rest of the tree.  So every shape gives
+// the same bits, and the shape is free to be tuned (run_batched measures the candidates per
+// size class of the graphs); the neighbour correction follows.  Results do not depend on
+// timing either.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "bl_internal.h"
@@ -28,9 +36,11 @@ namespace {
 
 constexpr int BT = 1024;       // threads per workgroup
 constexpr int MAXB = 1024;     // largest batch size of the batched path
+constexpr int NCH = 16;        // fixed j chunks of the dense sum (16 = BT / the smallest IPW)
 
 struct BatchedArgs {
     const int *order;          // graph ids, largest first
+    const int *ipw_of;         // batch slots served at once, per graph (power of two, 64..1024)
     int count;
     const int *voff;           // vertex offsets [count+1]
     const int *erp;            // effective-neighbour rowptr over all vertices (global entry ids)
@@ -65,6 +75,37 @@ __device__ __forceinline__ void dense_range(Src src, int j0, int j1, double2 pi,
     for (; u < j1; ++u) pair_rep<REFINE>(src[2 * u], src[2 * u + 1], pi.x, pi.y, ax, ay);
 }
 
+// Pairwise combination in a fixed tree: leaves arrive left to right, level l holds the pending
+// left subtree of 2^l leaves; after 2^L leaves (L <= 4) level L holds their sum,
+// ((v0 + v1) + (v2 + v3)) + ...  Named registers and uniform branches (the leaf count k is the
+// same in every lane), no indexed array that would go to scratch memory.
+struct PairTree {
+    double x0, y0, x1, y1, x2, y2, x3, y3, x4, y4;
+    int k = 0;
+    __device__ __forceinline__ void add(double vx, double vy) {
+        if (k & 1) {  // close the subtrees this leaf completes, left operand first
+            vx = x0 + vx, vy = y0 + vy;
+            if (k & 2) {
+                vx = x1 + vx, vy = y1 + vy;
+                if (k & 4) {
+                    vx = x2 + vx, vy = y2 + vy;
+                    if (k & 8) vx = x3 + vx, vy = y3 + vy;
+                }
+            }
+        }
+        if (!(k & 1)) x0 = vx, y0 = vy;
+        else if (!(k & 2)) x1 = vx, y1 = vy;
+        else if (!(k & 4)) x2 = vx, y2 = vy;
+        else if (!(k & 8)) x3 = vx, y3 = vy;
+        else x4 = vx, y4 = vy;
+        ++k;
+    }
+    __device__ __forceinline__ double2 total(int L) const {
+        return L == 0 ? make_double2(x0, y0) : L == 1 ? make_double2(x1, y1) : L == 2 ? make_double2(x2, y2)
+               : L == 3 ? make_double2(x3, y3) : make_double2(x4, y4);
+    }
+};
+
 template <bool REFINE>
 __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *red, double2 *fb, double *eb) {
     const int tid = threadIdx.x;
@@ -77,13 +118,11 @@ __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *r
         __syncthreads();
     }
     const int B = a.B;
-    const int bmax = min(B, n);
-    int ipw = 64;
-    while (ipw < bmax && ipw < BT) ipw <<= 1;
-    const int JS = BT / ipw;
+    const int ipw = __builtin_amdgcn_readfirstlane(a.ipw_of[g]);
+    const int JS = BT / ipw, cpt = NCH / JS;  // subgroups, chunks per subgroup
+    const int lcpt = __builtin_ctz(cpt), ljs = __builtin_ctz(JS);
     const int li = tid % ipw, js = tid / ipw;
-    const int sub = (n + JS - 1) / JS;
-    const int j0 = min(n, js * sub), j1 = min(n, j0 + sub);
+    const int sub = (n + NCH - 1) / NCH;      // chunk c: [c * sub, (c + 1) * sub) within [0, n)
     const int nb = (n + B - 1) / B;
     double step = a.step0, E = 0.0;
     for (int it = 0; it < a.iters; ++it) {
@@ -94,18 +133,23 @@ __device__ void layout_graph(const BatchedArgs &a, int g, double2 *P, double2 *r
                 const int slot = r0 + li;
                 const bool act = slot < bc;
                 const double2 pi = gp[b0 + (act ? slot : 0)];
-                double ax = 0.0, ay = 0.0;
-                if (in_lds)
-                    dense_range<REFINE>((lds_double *)P, j0, j1, pi, ax, ay);
-                else
-                    dense_range<REFINE>(reinterpret_cast<const double *>(gp), j0, j1, pi, ax, ay);
-                red[tid] = make_double2(ax, ay);
+                PairTree t;
+                for (int c = js * cpt; c < (js + 1) * cpt; ++c) {  // this subgroup's subtree
+                    const int j0 = min(n, c * sub), j1 = min(n, j0 + sub);
+                    double cx = 0.0, cy = 0.0;
+                    if (in_lds)
+                        dense_range<REFINE>((lds_double *)P, j0, j1, pi, cx, cy);
+                    else
+                        dense_range<REFINE>(reinterpret_cast<const double *>(gp), j0, j1, pi, cx, cy);
+                    t.add(cx, cy);
+                }
+                red[tid] = t.total(lcpt);
                 __syncthreads();
                 if (js == 0 && act) {
-                    for (int s = 1; s < JS; ++s) {
-                        ax += red[s * ipw + li].x;
-                        ay += red[s * ipw + li].y;
-                    }
+                    PairTree u;  // the subgroups' subtrees, in order: the top of the tree
+                    for (int s = 0; s < JS; ++s) u.add(red[s * ipw + li].x, red[s * ipw + li].y);
+                    const double2 tot = u.total(ljs);
+                    const double ax = tot.x, ay = tot.y;
                     const int i = v0 + b0 + slot;
                     double cx = 0.0, cy = 0.0;
                     for (int e = a.erp[i]; e < a.erp[i + 1]; ++e) {
@@ -171,6 +215,24 @@ __global__ __launch_bounds__(BT) void k_batched(BatchedArgs a) {
     }
 }
 
+// The fixed rule: every batch slot served in one round (the next power of two >= min(B, n)).
+int ipw_rule(int n, int B) {
+    const int bmax = std::min(B, n);
+    int ipw = 64;
+    while (ipw < bmax && ipw < BT) ipw <<= 1;
+    return ipw;
+}
+
+// Tuned shapes per (size class, B, precision, law, positions in LDS), for the process.
+struct TuneKey {
+    int cls, B, refine, law, lds;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(cls, B, refine, law, lds) < std::tie(o.cls, o.B, o.refine, o.law, o.lds);
+    }
+};
+std::mutex tune_mu;
+std::map<TuneKey, int> tuned;
+
 }  // namespace
 
 int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const int32_t *rowptr, const int32_t *colids,
@@ -221,10 +283,10 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
     buf.s = s;
     const size_t b_order = 4 * (size_t)count, b_voff = 4 * (size_t)(count + 1), b_erp = 4 * (size_t)(N + 1),
                  b_eci = 4 * std::max<size_t>(1, eci.size()), b_pos = 16 * (size_t)N, b_en = 8 * (size_t)count,
-                 b_fin = 8 * (size_t)count, b_wg = 8 * (size_t)grid, b_q = 64;
+                 b_fin = 8 * (size_t)count, b_wg = 8 * (size_t)grid, b_q = 64, b_ipw = 4 * (size_t)count;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t total = al(b_order) + al(b_voff) + al(b_erp) + al(b_eci) + al(b_pos) + al(b_en) + al(b_fin) +
-                         al(b_wg) + al(b_q);
+                         al(b_wg) + al(b_q) + 2 * al(b_ipw);
     BL_HIP(hipMalloc(&buf.p, total));
     char *c = static_cast<char *>(buf.p);
     int *d_order = reinterpret_cast<int *>(c); c += al(b_order);
@@ -235,7 +297,9 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
     double *d_en = reinterpret_cast<double *>(c); c += al(b_en);
     long long *d_fin = reinterpret_cast<long long *>(c); c += al(b_fin);
     long long *d_wg = reinterpret_cast<long long *>(c); c += al(b_wg);
-    int *d_q = reinterpret_cast<int *>(c);
+    int *d_q = reinterpret_cast<int *>(c); c += al(b_q);
+    int *d_ipw = reinterpret_cast<int *>(c); c += al(b_ipw);
+    int *d_torder = reinterpret_cast<int *>(c);  // a size class's sample graphs (tuning)
     BL_HIP(hipMemcpyAsync(d_order, order.data(), b_order, hipMemcpyHostToDevice, s));
     BL_HIP(hipMemcpyAsync(d_voff, voff, b_voff, hipMemcpyHostToDevice, s));
     BL_HIP(hipMemcpyAsync(d_erp, erp.data(), b_erp, hipMemcpyHostToDevice, s));
@@ -245,6 +309,7 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
 
     BatchedArgs a;
     a.order = d_order;
+    a.ipw_of = d_ipw;
     a.count = count;
     a.voff = d_voff;
     a.erp = d_erp;
@@ -265,6 +330,93 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
     hipEvent_t e0, e1;
     BL_HIP(hipEventCreate(&e0));
     BL_HIP(hipEventCreate(&e1));
+    struct Events {
+        hipEvent_t *a, *b;
+        ~Events() {
+            (void)hipEventDestroy(*a);
+            (void)hipEventDestroy(*b);
+        }
+    } evs{&e0, &e1};
+
+    // The shape of each graph: the fixed rule, or, for runs with enough work, the fastest of the
+    // candidate shapes per size class (graphs of 2^c .. 2^(c+1) - 1 vertices), measured once per
+    // process on up to TUNE_SAMPLE graphs of the class, one iteration each, on a copy of the
+    // positions.  All shapes give the same bits.
+    std::vector<int> ipw(count);
+    for (int k = 0; k < count; ++k) ipw[k] = ipw_rule(voff[k + 1] - voff[k], p->batch);
+    double work = 0.0;
+    for (int k = 0; k < count; ++k) work += (double)(voff[k + 1] - voff[k]) * (voff[k + 1] - voff[k]);
+    work *= std::max(1, p->iters);
+    const char *tenv = getenv("BL_TUNE");
+    const bool tune = !(tenv && !strcmp(tenv, "0")) && (work >= 2e9 || (tenv && !strcmp(tenv, "1")));
+    if (tune) {
+        constexpr int TUNE_SAMPLE = 24;
+        std::map<int, std::vector<int>> members;  // class -> graph ids in queue order
+        for (int q = 0; q < count; ++q) {
+            const int k = order[q], nk = voff[k + 1] - voff[k];
+            if (nk >= 2) members[31 - __builtin_clz((unsigned)nk)].push_back(k);
+        }
+        void *tpos = nullptr;
+        struct Free {
+            void **p;
+            ~Free() {
+                if (*p) (void)hipFree(*p);
+            }
+        } free_tpos{&tpos};
+        for (auto &cm : members) {
+            const std::vector<int> &ids = cm.second;
+            int nmax = 0;
+            for (int k : ids) nmax = std::max(nmax, voff[k + 1] - voff[k]);
+            const TuneKey key{cm.first, p->batch, a.refine, a.law, nmax <= cap ? 1 : 0};
+            int best = -1;
+            {
+                std::lock_guard<std::mutex> lock(tune_mu);
+                auto it = tuned.find(key);
+                if (it != tuned.end()) best = it->second;
+            }
+            const int top = ipw_rule(nmax, p->batch);
+            if (best < 0 && top > 64) {
+                if (!tpos) {
+                    BL_HIP(hipMalloc(&tpos, b_pos));
+                    BL_HIP(hipMemcpyAsync(tpos, d_pos, b_pos, hipMemcpyDeviceToDevice, s));
+                }
+                const int m = std::min((int)ids.size(), TUNE_SAMPLE);
+                BL_HIP(hipMemcpyAsync(d_torder, ids.data(), 4 * (size_t)m, hipMemcpyHostToDevice, s));
+                BatchedArgs t = a;
+                t.order = d_torder;
+                t.count = m;
+                t.iters = 1;
+                t.pos = static_cast<double2 *>(tpos);
+                float best_ms = 0.f;
+                for (int cand = 64; cand <= top; cand <<= 1) {
+                    std::vector<int> all(count, cand);
+                    BL_HIP(hipMemcpyAsync(d_ipw, all.data(), b_ipw, hipMemcpyHostToDevice, s));
+                    float ms = 0.f;
+                    for (int rep = 0; rep < 2; ++rep) {  // the second launch is timed
+                        BL_HIP(hipMemsetAsync(d_q, 0, b_q, s));
+                        BL_HIP(hipEventRecord(e0, s));
+                        hipLaunchKernelGGL(k_batched, dim3(std::min(m, grid)), dim3(BT), dyn, s, t);
+                        BL_HIP(hipGetLastError());
+                        BL_HIP(hipEventRecord(e1, s));
+                        BL_HIP(hipEventSynchronize(e1));
+                        BL_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    }
+                    if (best < 0 || ms < best_ms) best = cand, best_ms = ms;
+                }
+                std::lock_guard<std::mutex> lock(tune_mu);
+                tuned[key] = best;
+            }
+            if (best > 0)
+                for (int k : ids) ipw[k] = std::min(best, ipw_rule(voff[k + 1] - voff[k], p->batch));
+        }
+    }
+    if (const char *force = getenv("BL_BATCHED_IPW")) {  // one shape for every graph (tests)
+        const int f = atoi(force);
+        if (f >= 64 && f <= BT && (f & (f - 1)) == 0)
+            for (int k = 0; k < count; ++k) ipw[k] = std::min(f, ipw_rule(voff[k + 1] - voff[k], p->batch));
+    }
+    BL_HIP(hipMemcpyAsync(d_ipw, ipw.data(), b_ipw, hipMemcpyHostToDevice, s));
+    BL_HIP(hipMemsetAsync(d_q, 0, b_q, s));
     BL_HIP(hipEventRecord(e0, s));
     hipLaunchKernelGGL(k_batched, dim3(grid), dim3(BT), dyn, s, a);
     BL_HIP(hipGetLastError());
@@ -278,8 +430,6 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
     BL_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (energies) std::copy(en.begin(), en.end(), energies);
     if (finish_ms) {
         const long long t0 = *std::min_element(wg.begin(), wg.end());

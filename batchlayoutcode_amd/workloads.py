@@ -13,6 +13,8 @@
   delaunay2m-algo40  the same graph, monopole Barnes-Hut alone at -bht 0.4 (no model: a profile of the
                    plain path with nothing else in the process)
   batched4096      4,096 small graphs (64..8192 vertices, seed 4), algo 2, -batch 128, one launch
+  batched4096-pergraph  the same graphs laid out one after another, each by the single-graph path
+                   (bl_run_device on device-resident positions): the baseline of the one launch
 
 Initial coordinates are uniform in [0,1)^2 (seed 0).  Prints one JSON line per run.
 """
@@ -101,6 +103,25 @@ def main(argv=None):
         out.append(dict(workload=a.name, graphs=len(gs), vertices=N, iters=a.iters, seconds=st.seconds,
                         graphs_per_s=len(gs) / st.seconds, vertex_updates_per_s=N * a.iters / st.seconds,
                         finish_ms_p50=float(np.percentile(fin, 50)), finish_ms_p99=float(np.percentile(fin, 99))))
+    elif a.name == "batched4096-pergraph":
+        import torch
+
+        gl = graphs.batched_small(4096, seed=4)
+        total, N, t_prog = 0.0, 0, time.time()
+        for k, (nv, e) in enumerate(gl):
+            rp, ci = graphs.csr_from_edges(nv, e)
+            xy = torch.from_numpy(graphs.uniform_init(len(rp) - 1, k)).cuda()
+            torch.cuda.synchronize()
+            with api.Graph(rp, ci) as g:
+                st = g.run_device(xy.data_ptr(), algo=2, iters=a.iters, batch=128, precision=a.precision)
+            total += st.seconds
+            N += len(rp) - 1
+            if time.time() - t_prog > 20:
+                print(f"# {k + 1} graphs, {total:.1f} s of layout so far", file=sys.stderr, flush=True)
+                t_prog = time.time()
+        out.append(dict(workload=a.name, graphs=len(gl), vertices=N, iters=a.iters, seconds=total,
+                        graphs_per_s=len(gl) / total, vertex_updates_per_s=N * a.iters / total,
+                        note="sum of bl_run_device wall times (launch to synchronize), graph set-up excluded"))
     else:
         ap.error(f"unknown workload {a.name}")
     for rec in out:

@@ -86,3 +86,30 @@ def test_batched_rejects_bad_arguments(bl):
     big = graphs.csr_from_edges(*graphs.grid(100, 100))
     with pytest.raises(bl.BLError):
         bl.run_batched([big], [np.zeros((10000, 2))], iters=1, batch=128)
+
+
+def test_batched_shapes_give_the_same_bits(bl, monkeypatch):
+    """Every shape of a graph's batch step (64..1024 batch slots served at once, the rest of the
+    1024 threads splitting the j range) and the autotuner's choice: the same positions and
+    energies bit for bit, since the 16 fixed j chunks are always combined by the same pairwise
+    tree."""
+    rng = np.random.default_rng(21)
+    gs = _mixed_graphs(rng, [70, 300, 1000, 2500, 4000, 130, 900])
+    xys = [np.random.default_rng(400 + k).random((len(rp) - 1, 2)) for k, (rp, _) in enumerate(gs)]
+    runs = {}
+    for batch in (128, 1000):
+        for shape in ("64", "128", "256", "512", "1024", "tuned", "rule"):
+            for k in ("BL_BATCHED_IPW", "BL_TUNE"):
+                monkeypatch.delenv(k, raising=False)
+            if shape == "tuned":
+                monkeypatch.setenv("BL_TUNE", "1")
+            elif shape == "rule":
+                monkeypatch.setenv("BL_TUNE", "0")
+            else:
+                monkeypatch.setenv("BL_BATCHED_IPW", shape)
+            print(f"[case] batched shape {shape} batch {batch}", flush=True)
+            runs[shape] = bl.run_batched(gs, xys, algo=2, iters=2, batch=batch)
+        ref_xy, ref_en = runs["rule"][0], runs["rule"][1]
+        for shape, (xy, en, _, _) in runs.items():
+            assert all(np.array_equal(a, b) for a, b in zip(xy, ref_xy)), (batch, shape)
+            assert np.array_equal(en, ref_en), (batch, shape)
