@@ -1,6 +1,6 @@
 """Named layout workloads (the benchmark configurations) for timing and profiling runs.
 
-    python -m batchlayoutcode_amd.workloads <name> [--iters K] [--profile] [--precision P]
+    python -m batchlayoutcode_amd.workloads <name> [--iters K] [--profile] [--precision P] [--calib-copy]
 
   grid64-algo2     64x64 grid (4,096 vertices), algo 2, -batch 256
   rgg250k-algo2    random geometric graph, 250k points (seed 1), mean degree 8, LCC; algo 2, -batch 256
@@ -51,7 +51,17 @@ def main(argv=None):
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--precision", type=int, default=api.PREC_MIXED)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first copy 256 MiB device to device (the reference for scaling HBM counters)")
     a = ap.parse_args(argv)
+    if a.calib_copy:
+        import torch
+
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     out = []
     if a.name == "grid64-algo2":

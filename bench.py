@@ -96,10 +96,10 @@ def cpu_reference(n, rp, ci, init, batch, threads, algo, iters, bht=1.2):
                        f"flags); {secs:.2f} s")
 
 
-def measured_traffic():
-    """HBM bytes per k_pairs32 launch from the committed FETCH_SIZE / WRITE_SIZE runs, each
-    scaled by what the counter reads for a copy of known size (profiles/ holds the commands)."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_pairs32_rgg250k.json")
+def measured_traffic(name="traffic_k_pairs32_rgg250k.json"):
+    """HBM bytes per launch of a kernel from the committed FETCH_SIZE / WRITE_SIZE runs, each
+    scaled by what the counter reads for a copy of known size (the file holds the commands)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     t = json.load(open(path))
@@ -230,12 +230,16 @@ def c4_roofline(n, nnz, lp):
     # the batch walk: attraction over every CSR entry with live positions, then the move
     bytes_per_iter = n * (4 + 16 + 16 + 16 + 8) + nnz * (4 + 16)
     achieved = bytes_per_iter / (walk_ms / 1e3) / 1e9
+    traffic = measured_traffic("traffic_k_walk_lag_delaunay2m.json") if sp.walk == 6 else None
     return dict(bound="hbm", kernel=WALK_KERNELS.get(sp.walk, "walk"), achieved=achieved, peak=PEAK_HBM_GBS, unit="GB/s",
-                frac=achieved / PEAK_HBM_GBS, traffic=None, mean_launch_us=walk_ms * 1e3,
+                frac=achieved / PEAK_HBM_GBS, traffic=traffic, mean_launch_us=walk_ms * 1e3,
                 launches=sp.kernel_launches[1], bytes_per_launch=bytes_per_iter,
                 note="algorithmic bytes of one iteration's attraction and move (4 B rowptr + 16 B own position + "
                      "16 B repulsion + 16 B written position + 8 B energy term per vertex, 4 B column id + 16 B "
-                     "neighbour position per CSR entry) / mean time of the walk and its pre-pass")
+                     "neighbour position per CSR entry) / mean time of the walk and its pre-pass; traffic: their "
+                     "HBM bytes per iteration from FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size "
+                     "(profiles/traffic_k_walk_lag_delaunay2m.json); the walk is bound by its per-batch latency "
+                     "(one workgroup walks the 6,497 sequential batches), not by bytes")
 
 
 def algo2_leg(args, torch, api, graphs):
