@@ -201,7 +201,9 @@ def c4_leg(args, torch, api, graphs):
             res[name] = dict(theta=q["theta"], value=n * K / secs, ms_per_step=secs * 1e3 / K, energy=st.energy,
                              kernel_ms_per_iter=per_iter, far_cells_per_iter=sp.far_cells, walk=sp.walk,
                              walk_lag=sp.walk_lag, walk_batches_helped=sp.walk_helped,
-                             model_iterations=sp.far_model_iters if algo == api.ALGO_FAR_LEARNED else None)
+                             model_iterations=sp.far_model_iters if algo == api.ALGO_FAR_LEARNED else None,
+                             judge_median_errors=(dict(learned=sp.far_judge_err[0], analytic=sp.far_judge_err[1])
+                                                  if algo == api.ALGO_FAR_LEARNED else None))
             log(f"{name} timed at theta {q['theta']}: {K} iterations in {secs:.3f} s = {n * K / secs:.3e} v-u/s; "
                 f"per iteration {json.dumps({k: round(v, 3) for k, v in per_iter.items()})} ms; "
                 f"cells {sp.far_cells}")
