@@ -45,7 +45,7 @@ class bl_stats(C.Structure):
                 ("seconds", C.c_double), ("step_final", C.c_double), ("kernel_ms", C.c_double * 8),
                 ("kernel_launches", C.c_int32 * 8), ("walk", C.c_int32), ("far_cells", C.c_int64),
                 ("far_model_iters", C.c_int32), ("far_judge_err", C.c_float * 2), ("walk_helped", C.c_int32),
-                ("walk_lag", C.c_int32)]
+                ("walk_lag", C.c_int32), ("nonfinite", C.c_int32)]
 
 
 _lib = None
@@ -125,12 +125,13 @@ class Stats:
     far_judge_err: tuple = (0.0, 0.0)
     walk_helped: int = 0
     walk_lag: int = 0
+    nonfinite: int = 0
 
 
 def _stats(st: bl_stats) -> Stats:
     return Stats(st.energy, st.iters_done, st.seconds, st.step_final, tuple(st.kernel_ms),
                  tuple(st.kernel_launches), st.walk, st.far_cells, st.far_model_iters, tuple(st.far_judge_err),
-                 st.walk_helped, st.walk_lag)
+                 st.walk_helped, st.walk_lag, st.nonfinite)
 
 
 class Graph:

@@ -174,7 +174,7 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
     if (!g) return BL_E_ARG;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     DevBuf *all[] = {&g->rowptr, &g->colids, &g->w_edge, &g->eff_rowptr, &g->eff_colids, &g->eff_w,
-                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->far_model, &g->cpre, &g->cpv,
+                     &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->guard, &g->far_model, &g->cpre, &g->cpv,
                      &g->early_rp, &g->early_ci, &g->early_w, &g->late_rp, &g->late_ci, &g->late_w,
                      &g->grid_rows, &g->lag_erp, &g->lag_esp, &g->lag_eci, &g->lag_ew, &g->lag_rofs, &g->lag_rw,
                      &g->lag_pre, &g->lag_ready};
@@ -251,6 +251,15 @@ static int run_on_device(bl_graph *g, double *d_xy, const bl_params *p, bl_stats
             break;
         default:
             rc = BL_E_ARG;
+    }
+    if (rc == BL_OK) {  // the non-finite guard
+        if (g->guard.ensure(64)) {
+            rc = BL_E_NOMEM;
+        } else {
+            int nf = 0;
+            rc = count_nonfinite(reinterpret_cast<const double2 *>(d_xy), g->n, g->guard.as<int>(), g->stream, &nf);
+            local.nonfinite = nf;
+        }
     }
     local.status = rc;
     if (st) *st = local;

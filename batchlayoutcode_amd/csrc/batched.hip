@@ -423,6 +423,9 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
     BL_HIP(hipEventRecord(e1, s));
     std::vector<double> en(count);
     std::vector<long long> fin(count), wg(grid);
+    int nonfinite = 0;  // the guard's counter in the queue block's spare words
+    int rc = count_nonfinite(d_pos, N, d_q + 8, s, &nonfinite);
+    if (rc) return rc;
     BL_HIP(hipMemcpyAsync(xy, d_pos, b_pos, hipMemcpyDeviceToHost, s));
     BL_HIP(hipMemcpyAsync(en.data(), d_en, b_en, hipMemcpyDeviceToHost, s));
     BL_HIP(hipMemcpyAsync(fin.data(), d_fin, b_fin, hipMemcpyDeviceToHost, s));
@@ -445,6 +448,7 @@ int run_batched(int32_t count, const int32_t *voff, const int64_t *eoff, const i
         st->step_final = sstep;
         st->kernel_ms[0] = ms;
         st->kernel_launches[0] = 1;
+        st->nonfinite = nonfinite;
     }
     return BL_OK;
 }

@@ -93,7 +93,7 @@ struct bl_graph {
     std::vector<int32_t> h_cpre;
     int32_t corr_max_pieces = 1;
     // workspace
-    bl::DevBuf pos, part, eterm, energy, scratch;
+    bl::DevBuf pos, part, eterm, energy, scratch, guard;
     bl::BHWork bh;
     bl::DevBuf far_model;  // learned far-field parameters (bl_graph_set_far_model)
     bool has_far_model = false;
@@ -210,6 +210,9 @@ inline void effective_neighbors(const int32_t *rowptr, const int32_t *colids, co
 }
 
 // minibatch.hip
+// The number of the n positions that are not finite, counted on the device into *d_count (one
+// int of device memory) and returned in *h_count (waits for the stream).
+int count_nonfinite(const double2 *pos, int n, int *d_count, hipStream_t s, int *h_count);
 // static_ws: the workspace in the library's static device array instead of the graph's buffers
 // (allocates nothing; BL_E_SCRATCH when it does not fit)
 int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double decay, double step0,

@@ -239,6 +239,32 @@ __global__ void k_step_decay(double *stepp, double decay) {
 
 }  // namespace
 
+// ---- the non-finite guard -------------------------------------------------------------
+__global__ void k_zero_int(int *p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
+// One vector atomic per wave that holds a non-finite coordinate.
+__global__ __launch_bounds__(256) void k_count_nonfinite(const double2 *__restrict__ pos, int n, int *count) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    bool bad = false;
+    if (i < n) {
+        const double2 p = pos[i];
+        bad = !isfinite(p.x) || !isfinite(p.y);
+    }
+    const uint64_t m = __ballot(bad);
+    if (m && (threadIdx.x & 63) == (int)(__builtin_ctzll(m))) atomicAdd(count, __popcll(m));
+}
+
+int count_nonfinite(const double2 *pos, int n, int *d_count, hipStream_t s, int *h_count) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, d_count);
+    if (n > 0) hipLaunchKernelGGL(k_count_nonfinite, dim3((n + 255) / 256), dim3(256), 0, s, pos, n, d_count);
+    BL_HIP(hipGetLastError());
+    BL_HIP(hipMemcpyAsync(h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, s));
+    BL_HIP(hipStreamSynchronize(s));
+    return BL_OK;
+}
+
 // ---- energy reductions (shared with the Barnes-Hut path) -------------------------------
 // Fixed-order tree sum of |F_i|^2 (FP32 mode).
 // Energy sum_i |F_i|^2 in two fixed-order stages: ENERGY_PARTS workgroups each add a
@@ -493,6 +519,12 @@ int run_minibatch(bl_graph *g, double *d_xy, const bl_params *p, int law, double
         st->energy = loop > 0 ? E : (law == LAW_FR ? 0.0 : 1.79769313486231570815e+308);
         st->iters_done = loop;
         st->step_final = STEP;
+        if (static_ws) {  // the guard's counter in the energy block's spare word
+            int nf = 0;
+            const int rc = count_nonfinite(pos, n, reinterpret_cast<int *>(energy + 3), s, &nf);
+            if (rc) return rc;
+            st->nonfinite = nf;
+        }
     }
     return BL_OK;
 }

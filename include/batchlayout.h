@@ -115,6 +115,10 @@ typedef struct {
        helper workgroups (the walker formed the others itself), and the lag D it ran with */
     int32_t walk_helped;
     int32_t walk_lag;
+    /* vertices whose final position is not finite (NaN or infinite).  The reference's move
+       divides by |F| (Coordinate.h:63-65), so a vertex with F = 0 becomes NaN and the NaN
+       spreads to its neighbours; the layout keeps that behaviour and reports it here */
+    int32_t nonfinite;
 } bl_stats;
 
 /* The batch walks of the Barnes-Hut family (bl_stats.walk); the library picks one per graph and

@@ -354,3 +354,25 @@ def test_lag_walk_declines_crowded_recent_batches(bl, monkeypatch):
     assert d_fit < 20
     for lag in ("6", None):
         assert np.abs(outs[lag] - outs["20"]).max() <= 1e-10 * diameter(outs["20"])
+
+
+@pytest.mark.parametrize("algo", [2, 4, 40])
+def test_nonfinite_guard(bl, oracle, algo):
+    """A vertex whose force is exactly zero (a graph of one vertex, or a vertex placed where every
+    force cancels) moves to NaN, as the reference's x += STEP F/|F| does (Coordinate.h:63-65); the
+    run keeps that result and reports the vertices whose final position is not finite."""
+    one = (np.array([0, 0], np.int32), np.zeros(0, np.int32))
+    print(f"[case] non-finite guard algo {algo}", flush=True)
+    with bl.Graph(*one) as g:
+        xy, st = g.run(np.array([[0.3, 0.7]]), algo=algo, iters=2, batch=16, bht=1.2)
+    assert np.isnan(xy).all() and st.nonfinite == 1
+    if algo == 2:
+        ref, _, _ = oracle.minibatch(*one, np.array([[0.3, 0.7]]), 2, 16, law=2)
+        assert np.isnan(ref).all()
+    rp, ci = graphs.csr_from_edges(*graphs.grid(8, 8))
+    with bl.Graph(rp, ci) as g:
+        xy, st = g.run(np.random.default_rng(1).random((64, 2)), algo=algo, iters=3, batch=16, bht=1.2)
+    assert np.isfinite(xy).all() and st.nonfinite == 0
+    out, en, _, stb = bl.run_batched([one, (rp, ci)], [np.array([[0.3, 0.7]]), np.random.default_rng(1).random((64, 2))],
+                                      algo=2, iters=2, batch=16)
+    assert np.isnan(out[0]).all() and np.isfinite(out[1]).all() and stb.nonfinite == 1
