@@ -203,7 +203,8 @@ __device__ __forceinline__ void far_features(const FarCell &cl, double dx, doubl
 // The same features in FP32 with hardware reciprocal, reciprocal square root and log2 (about
 // 1 ulp each): the learned traversal's model takes them as BF16 inputs, so FP32 is ample,
 // and FP64 divisions, square root and log2 cost every wave in which any lane accepts a cell.
-// Training samples (k_far_samples) and the quadrupole term use far_features.
+// Training samples (k_far_samples) use far_features; the quadrupole term has its own closed
+// form in d (k_far_trav).
 __device__ __forceinline__ void far_features32(const FarCell &cl, double dx, double dy, double r2, double D,
                                                float *f) {
     const float irho = __builtin_amdgcn_rsqf((float)r2);
@@ -332,16 +333,21 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         const bool acc = D * D < th2 * r2;
         if (ncells) accepted += (unsigned)__popcll(__ballot(act && acc));
         if (act && acc) {
-            const double s = (double)cl.m * inv_r2<true>(dx, dy);
-            const double mx = dx * s, my = dy * s;
-            fx += mx;
-            fy += my;
+            const double ir2 = inv_r2<true>(dx, dy);
+            const double s = (double)cl.m * ir2;
+            fx += dx * s;
+            fy += dy * s;
             if (quad) {
-                float f[8];
-                far_features(cl, dx, dy, r2, D, f);
-                const double a = (double)f[4], b = -2.0 * (double)f[2];
-                fx += a * mx - b * my;
-                fy += a * my + b * mx;
+                // the second-order term (a, b) = (S_uu - S_ww, -2 S_uw) applied to the monopole
+                // m d / |d|^2: in d itself the mass cancels and it is |d|^-6 (A d + B d_perp),
+                // A = (dx^2 - dy^2)(Mxx - Myy) + 4 dx dy Mxy, B = -2 (dx dy (Myy - Mxx) +
+                // (dx^2 - dy^2) Mxy): FP64 products only, no square root, division or log
+                const double ex = dx * dx - dy * dy, xy = dx * dy;
+                const double A = fma(ex, cl.mxx - cl.myy, 4.0 * xy * cl.mxy);
+                const double Bq = -2.0 * fma(xy, cl.myy - cl.mxx, ex * cl.mxy);
+                const double i6 = ir2 * ir2 * ir2;
+                fx = fma(i6, fma(A, dx, -Bq * dy), fx);
+                fy = fma(i6, fma(A, dy, Bq * dx), fy);
             }
         }
         if (MODE == TRAV_LEARNED && learned) {
