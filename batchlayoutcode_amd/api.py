@@ -25,7 +25,7 @@ FAR_MODEL_PARAMS, FAR_FEATURES = 9026, 8
 PREC_MIXED, PREC_FP64, PREC_FP32 = 0, 1, 2
 BATCHED_MAX_N = 8192
 # the Barnes-Hut family's batch walks (Stats.walk)
-WALK_NONE, WALK_ROW_ORDER, WALK_PF, WALK_SPLIT, WALK_ROW, WALK_GRID, WALK_LAG = 0, 1, 2, 3, 4, 5, 6
+WALK_NONE, WALK_ROW_ORDER, WALK_PF, WALK_SPLIT, WALK_ROW, WALK_GRID, WALK_LAG, WALK_WIDE = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 class BLError(RuntimeError):

@@ -1492,29 +1492,6 @@ struct GridWalk {
     long long ticks;
 };
 
-__device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-
-// One step of a segmented inclusive scan over the wave by a DPP move: each lane takes the
-// value and key of the lane CTRL names (row_shr:n within rows of 16 lanes; row_bcast:15 /
-// row_bcast:31, the last lane of the previous row / of the first two rows, into the rows
-// of ROW_MASK) and adds the value when the keys are equal.  Lanes without a source lane
-// see the key -2, which no lane holds.  After the steps 1, 2, 4, 8, bcast15, bcast31 lane l
-// holds the sum of the values of its run (the lanes before it with its key) up to l:
-// runs are contiguous, so a run that reaches back into an earlier row of 16 contains that
-// row's last lane.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ void seg_step(double2 &v, int key) {
-    const int ks = __builtin_amdgcn_update_dpp(-2, key, CTRL, ROW_MASK, 0xF, false);
-    const int xl = __builtin_amdgcn_update_dpp(0, __double2loint(v.x), CTRL, ROW_MASK, 0xF, false);
-    const int xh = __builtin_amdgcn_update_dpp(0, __double2hiint(v.x), CTRL, ROW_MASK, 0xF, false);
-    const int yl = __builtin_amdgcn_update_dpp(0, __double2loint(v.y), CTRL, ROW_MASK, 0xF, false);
-    const int yh = __builtin_amdgcn_update_dpp(0, __double2hiint(v.y), CTRL, ROW_MASK, 0xF, false);
-    if (ks == key) {
-        v.x += __hiloint2double(xh, xl);
-        v.y += __hiloint2double(yh, yl);
-    }
-}
-
 __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
     constexpr int T = GW_T;
     constexpr int PR = (WALK_MAXB + T) / T;      // rowptr values per thread (a batch has bc + 1)
@@ -1634,7 +1611,7 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
                             hr = r;
                             hv = make_double2(ax, ay);
                         } else {
-                            a.fbuf[r] = make_double2(ax, ay);
+                            store_through(a.fbuf + r, make_double2(ax, ay));
                         }
                         ++r;
                         rend = rps[r + 1 - b0];
@@ -1688,8 +1665,8 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
         if (Sw < Ew) {
             const int rf = frf, rl = frl;  // row_of(Sw), row_of(Ew - 1)
             if (t == 0) {
-                a.wfirst[par * GW_G + w] = rf;
-                a.wlast[par * GW_G + w] = rl != rf ? rl : -1;
+                store_through(a.wfirst + par * GW_G + w, rf);
+                store_through(a.wlast + par * GW_G + w, rl != rf ? rl : -1);
             }
             for (int i = rf + t; i <= rl; i += T) {
                 const int r0 = rps[i - b0], r1 = rps[i + 1 - b0];
@@ -1699,11 +1676,11 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
                 if (ta == tb && hrow[ta] != i && trow[ta] != i) continue;  // already in fbuf
                 const double2 val = hrow[ta] == i ? S[tb] : (tb > ta ? add2(tail[ta], S[tb]) : tail[ta]);
                 if (r0 >= Sw && r1 <= Ew) {
-                    a.fbuf[i] = val;
+                    store_through(a.fbuf + i, val);
                 } else if (i == rf) {
-                    a.whead[par * GW_G + w] = val;
+                    store_through(a.whead + par * GW_G + w, val);
                 } else {
-                    a.wtail[par * GW_G + w] = val;
+                    store_through(a.wtail + par * GW_G + w, val);
                 }
             }
         }
@@ -1728,14 +1705,14 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
                 for (int g = 0; g < GW_K; ++g) wf[g] = a.w_edge[max(0, min(ns + g, ne - 1))];
             }
         }
-        // grid barrier: every workgroup's share of the batch is summed.  Each wave waits for
-        // its stores to reach the L2; thread 0's release below writes the L2 back, so one
-        // agent-scope fence per workgroup publishes the stores of all its waves (a fence in
-        // every wave would write the L2 back sixteen times).
+        // grid barrier: every workgroup's share of the batch is summed.  What other workgroups
+        // read (row sums, partials, moved positions) was stored through to memory; each wave
+        // waits for its stores to complete, so thread 0's relaxed arrival publishes them with no
+        // L2 write-back (a release here wrote the whole L2 back at every batch).
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         if (t == 0) {
-            __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const unsigned target = (unsigned)G * (unsigned)(b + 1);
             const long long t0 = wall_clock64();
             // polled relaxed (an acquiring load would invalidate the L2 at every poll), then
@@ -1799,14 +1776,16 @@ __global__ __launch_bounds__(GW_T) void k_walk_grid(GridWalk a) {
             double2 f = rps[i + 1 - b0] > rps[i - b0] ? fv[k] : make_double2(0.0, 0.0);
             f.x -= rv[k].x;  // f - calcRepForce(x_i), :1059
             f.y -= rv[k].y;
-            const double mag = sqrt(f.x * f.x + f.y * f.y);
+            // x_i += (F/|F|) * STEP (:1062-1066), the unit vector F * rsqrt2(|F|^2)
+            const double m2 = f.x * f.x + f.y * f.y;
+            const double rs = rsqrt2(m2) * a.step;
             double2 q = bpos[i - b0];
-            q.x = q.x + (f.x / mag) * a.step;
-            q.y = q.y + (f.y / mag) * a.step;
+            q.x = fma(f.x, rs, q.x);
+            q.y = fma(f.y, rs, q.y);
             newpos[i - b0] = q;
             if (w == 0) {
-                a.pos[i] = q;
-                a.eterm[i] = f.x * f.x + f.y * f.y;
+                store_through(a.pos + i, q);
+                a.eterm[i] = m2;
             }
         }
         __syncthreads();
@@ -2056,13 +2035,26 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
     bool use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB &&
                     (double)g->nnz / (double)nbt >= (double)GW_MIN_ENTRIES;
     bool use_row = p->precision != BL_PREC_FP64 && row_walk_fits(g, p->batch);
+    // where the multi-workgroup walk would run, the wide lagged walk (helper workgroups) runs
+    // instead when its plan fits; BL_WALK=grid keeps the multi-workgroup walk, BL_WALK=wide
+    // takes the wide walk wherever it fits
+    bool want_wide = use_grid;
     if (const char *wenv = getenv("BL_WALK")) {
-        if (!strcmp(wenv, "grid")) use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB;
-        else if (!strcmp(wenv, "single")) use_grid = false;
-        else if (!strcmp(wenv, "pf")) use_grid = false, use_row = false;
-        else if (!strcmp(wenv, "row")) use_grid = false;
+        if (!strcmp(wenv, "grid")) use_grid = p->precision != BL_PREC_FP64 && p->batch <= WALK_MAXB, want_wide = false;
+        else if (!strcmp(wenv, "single")) use_grid = false, want_wide = false;
+        else if (!strcmp(wenv, "pf")) use_grid = false, use_row = false, want_wide = false;
+        else if (!strcmp(wenv, "row")) use_grid = false, want_wide = false;
+        else if (!strcmp(wenv, "wide")) want_wide = p->precision != BL_PREC_FP64;
     }
-    if (use_grid) use_row = false;
+    const bool use_wide = want_wide && wide_walk_fits(g, p->batch);
+    int wide_h = 255;
+    if (const char *v = getenv("BL_WIDE_H")) wide_h = std::max(1, atoi(v));
+    if (use_wide) {
+        rc = wide_walk_prepare(g->has_w);
+        if (rc) return rc;
+        use_grid = false;
+    }
+    if (use_grid || use_wide) use_row = false;
     // the lagged walk (helper workgroups) where the row walk fits and the graph has enough batches;
     // BL_LAG_D / BL_LAG_H set its lag and helper count
     int lag_d = 0, lag_h = 32;
@@ -2143,6 +2135,10 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
                                g->has_w ? g->w_edge.as<float>() : nullptr, p->weight, pos, g->bh.rep.as<double2>(),
                                g->scratch.as<double2>(), n, p->batch, STEP, g->eterm.as<double>(),
                                g->bh.misc.as<int>());
+        } else if (use_wide) {
+            walk = BL_WALK_WIDE;
+            rc = launch_walk_wide(g, pos, p, STEP, wide_h, s);
+            if (rc) return rc;
         } else if (use_grid && launch_walk_grid(g, pos, p, STEP, s) == BL_OK) {
             walk = BL_WALK_GRID;
         } else if (use_lag) {
@@ -2194,16 +2190,22 @@ int run_bh(bl_graph *g, double *d_xy, const bl_params *p, double step0, int iter
         BL_HIP(hipMemcpyAsync(&status, g->bh.misc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, s));
         BL_HIP(hipMemcpyAsync(&E, g->energy.as<double>(), sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    int gerr = 0;
+    int gerr = 0, werr = 0, waited = 0;
     if (g->bh.gwalk_used) BL_HIP(hipMemcpyAsync(&gerr, gwalk_err(g->bh), sizeof(int), hipMemcpyDeviceToHost, s));
+    if (walk == BL_WALK_WIDE && (rc = wide_walk_read(g, &werr, &waited, s))) return rc;
     BL_HIP(hipStreamSynchronize(s));
     if (status & ST_DUPKEY) return BL_E_DUPKEY;
     if (gerr) return BL_E_DEVICE;  // a grid barrier timed out (the walk stopped early)
+    if (werr) {  // a wait of the wide walk timed out: its chunk counts are incomplete
+        (void)wide_walk_reset(g);
+        return BL_E_DEVICE;
+    }
+    if (walk == BL_WALK_WIDE) helped = nbt - waited;
     if (st) {
         st->far_cells = (int64_t)cells;
         st->far_model_iters = model_iters;
         st->walk_helped = helped;
-        st->walk_lag = walk == BL_WALK_LAG ? lag_d : 0;
+        st->walk_lag = walk == BL_WALK_LAG ? lag_d : walk == BL_WALK_WIDE ? g->wide_dmax : 0;
         st->far_judge_err[0] = judge_err[0];
         st->far_judge_err[1] = judge_err[1];
         st->energy = E;

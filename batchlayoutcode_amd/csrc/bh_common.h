@@ -17,6 +17,64 @@ struct BHNode {
     int cnt;       // bottom-up arrival counter
 };
 
+// 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, relative error about 2^-25) refined by
+// two Newton steps y <- y (3/2 - (x/2) y^2), within a few ulp of the correctly rounded value;
+// nine dependent operations where an IEEE square root and a division take about twenty-five.
+// x = 0 gives NaN (0 * inf), as the reference's 0 / 0 does.
+__device__ __forceinline__ double rsqrt2(double x) {
+    const double h = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+// Stores for another workgroup with agent-scope stores, which the XCD's L2 writes through to
+// memory: once they have completed (s_waitcnt), a relaxed flag or counter published after them
+// needs no L2 write-back, and a reader that acquired after seeing it reads the new values.
+__device__ __forceinline__ void store_through(double2 *p, double2 v) {
+    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_through(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// One step of a segmented inclusive scan over the wave by a DPP move: each lane takes the
+// value and key of the lane CTRL names (row_shr:n within rows of 16 lanes; row_bcast:15 /
+// row_bcast:31, the last lane of the previous row / of the first two rows, into the rows
+// of ROW_MASK) and adds the value when the keys are equal.  Lanes without a source lane
+// see the key -2, which no lane holds.  After the steps 1, 2, 4, 8, bcast15, bcast31 lane l
+// holds the sum of the values of its run (the lanes before it with its key) up to l:
+// runs are contiguous, so a run that reaches back into an earlier row of 16 contains that
+// row's last lane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_step(double2 &v, int key) {
+    const int ks = __builtin_amdgcn_update_dpp(-2, key, CTRL, ROW_MASK, 0xF, false);
+    const int xl = __builtin_amdgcn_update_dpp(0, __double2loint(v.x), CTRL, ROW_MASK, 0xF, false);
+    const int xh = __builtin_amdgcn_update_dpp(0, __double2hiint(v.x), CTRL, ROW_MASK, 0xF, false);
+    const int yl = __builtin_amdgcn_update_dpp(0, __double2loint(v.y), CTRL, ROW_MASK, 0xF, false);
+    const int yh = __builtin_amdgcn_update_dpp(0, __double2hiint(v.y), CTRL, ROW_MASK, 0xF, false);
+    if (ks == key) {
+        v.x += __hiloint2double(xh, xl);
+        v.y += __hiloint2double(yh, yl);
+    }
+}
+
+// One attraction term of vertex v toward pj, d * (Wk |d|) (algorithms.cpp:1055-1058), added to
+// (ax, ay), with |d| = |d|^2 / sqrt(|d|^2) from rsqrt2 (coincident or nearly coincident vertices,
+// |d| < 2^-450, add nothing, where the reference adds at most 2^-900 W).  The walks with helper
+// workgroups use it in the walker and in the helpers alike, so their sums agree bit for bit.
+__device__ __forceinline__ void attr_add(double2 pj, double2 v, double Wk, double &ax, double &ay) {
+    const double dx = pj.x - v.x, dy = pj.y - v.y;
+    const double d2 = dx * dx + dy * dy;
+    const double sc = d2 > 0x1p-900 ? Wk * (d2 * rsqrt2(d2)) : 0.0;
+    ax = fma(dx, sc, ax);
+    ay = fma(dy, sc, ay);
+}
+
 // status word bits (misc[0])
 constexpr int ST_DUPKEY = 1;
 
@@ -27,10 +85,6 @@ constexpr int ST_DUPKEY = 1;
 // itself (far_cells_of).
 int bh_build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s);
 
-// Repulsion of every vertex for the physically consistent layouts (farfield.hip): into
-// w.rep (by vertex id).  FAR_EXACT sums over all points; the other modes read the tree of
-// d_pos already built in w by bh_build_tree.  FAR_LEARNED needs model (device, MLP_NPARAM
-// floats).
 // The lagged batch walk (walk_lag.hip): lag_walk_fits builds the settled / recent split for
 // batch size B and lag D (false: the walk does not take this graph and batch size);
 // launch_walk_lag runs one iteration's walk with `helpers` helper workgroups (after
@@ -41,7 +95,22 @@ int launch_walk_lag(bl_graph *g, double2 *pos, const bl_params *p, double step, 
 int lag_walk_prepare(bool has_w);
 int *lag_walk_helped(bl_graph *g);
 
-// The other modes need the cell summaries of that tree first (far_cells_of).  ncells
+// The wide lagged batch walk (walk_wide.hip): wide_walk_fits builds the near / far split and
+// the helpers' chunks for batch size B (largest lag from BL_WIDE_D; false: the walk does not
+// take this graph and batch size); launch_walk_wide runs one iteration's walk (no pre-pass)
+// with up to `helpers` helper workgroups; wide_walk_read queues the copies of the error flag
+// and of the number of batches the walker waited for (valid after the stream synchronizes);
+// wide_walk_prepare sets the kernels' LDS limit.
+bool wide_walk_fits(bl_graph *g, int B);
+int launch_walk_wide(bl_graph *g, double2 *pos, const bl_params *p, double step, int helpers, hipStream_t s);
+int wide_walk_read(bl_graph *g, int *err, int *waited, hipStream_t s);
+int wide_walk_reset(bl_graph *g);
+int wide_walk_prepare(bool has_w);
+
+// Repulsion of every vertex for the physically consistent layouts (farfield.hip): into
+// w.rep (by vertex id).  FAR_EXACT sums over all points; the other modes read the tree of
+// d_pos already built in w by bh_build_tree.  FAR_LEARNED needs model (device, MLP_NPARAM
+// floats).  The other modes need the cell summaries of that tree first (far_cells_of).  ncells
 // (device, nullable) is incremented by the number of cells the traversal accepts.
 enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3, FAR_QUAD = 4 };
 int far_cells_of(BHWork &w, int n, hipStream_t s);

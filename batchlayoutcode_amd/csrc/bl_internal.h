@@ -83,6 +83,14 @@ struct bl_graph {
     int32_t lag_auto_batch = 0, lag_auto_d = 0;  // the default lag for batch size lag_auto_batch
     bool lag_ok = false;
     bl::DevBuf lag_erp, lag_esp, lag_eci, lag_ew, lag_rofs, lag_rw, lag_pre, lag_ready;
+    // the wide lagged walk (walk_wide.hip) at batch size wide_batch and largest lag wide_dmax:
+    // each row's entries split into near ones (the walker's, with ring slots) and far ones (the
+    // helpers', in chunks of whole rows), the walker's per-thread tables, the helpers' sums and
+    // per-batch chunk counts (summed over launches: wide_gen * chunks when a batch is complete)
+    int32_t wide_batch = 0, wide_dmax = 0, wide_gen = 0, wide_nchunks = 0, wide_maxnch = 0, wide_cap = 0;
+    bool wide_ok = false;
+    bl::DevBuf wide_near_rp, wide_near_code, wide_near_w, wide_npa, wide_npb, wide_ring_of, wide_far_rp,
+        wide_far_ci, wide_far_w, wide_chunks, wide_cfr, wide_nch, wide_pre, wide_cnt, wide_sink;
     // the neighbours the reference's merge pointer actually meets (algorithms.cpp:758-762),
     // as a CSR of strictly increasing column ids per row (+ their weights)
     bl::DevBuf eff_rowptr, eff_colids, eff_w;

@@ -147,30 +147,6 @@ __device__ __forceinline__ void wbar() {
     __asm__ volatile("" ::: "memory");
 }
 
-// 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, relative error about 2^-25) refined by
-// two Newton steps y <- y (3/2 - (x/2) y^2), within a few ulp of the correctly rounded value.
-// Nine dependent operations where an IEEE square root and a division take about twenty-five.
-// x = 0 gives NaN (0 * inf), as the reference's 0 / 0 does.
-__device__ __forceinline__ double rsqrt2(double x) {
-    const double h = 0.5 * x;
-    double y = __builtin_amdgcn_rsq(x);
-    y = y * fma(-h * y, y, 1.5);
-    y = y * fma(-h * y, y, 1.5);
-    return y;
-}
-
-// One attraction term of vertex v toward pj, d * (Wk |d|) (algorithms.cpp:1055-1058), added to
-// (ax, ay), with |d| = |d|^2 / sqrt(|d|^2) from rsqrt2 (coincident or nearly coincident vertices,
-// |d| < 2^-450, add nothing, where the reference adds at most 2^-900 W).  The walker and the
-// helpers both use it, so their pre_i agree bit for bit.
-__device__ __forceinline__ void attr_add(double2 pj, double2 v, double Wk, double &ax, double &ay) {
-    const double dx = pj.x - v.x, dy = pj.y - v.y;
-    const double d2 = dx * dx + dy * dy;
-    const double sc = d2 > 0x1p-900 ? Wk * (d2 * rsqrt2(d2)) : 0.0;
-    ax = fma(dx, sc, ax);
-    ay = fma(dy, sc, ay);
-}
-
 // Column ids (and weights) of the first LAG_HS settled entries [e0, e1) (clamped loads).
 template <bool HAS_W>
 __device__ __forceinline__ void settled_ids(const LagArgs &a, int e0, int e1, int *c, float *w) {

@@ -112,7 +112,9 @@ typedef struct {
     int32_t far_model_iters;
     float far_judge_err[2];
     /* BL_WALK_LAG: batches of the last iteration whose sums over long-moved neighbours came from the
-       helper workgroups (the walker formed the others itself), and the lag D it ran with */
+       helper workgroups (the walker formed the others itself), and the lag D it ran with.
+       BL_WALK_WIDE: batches of the last iteration whose helper sums were ready when the walker
+       reached them (it waited for the others), and the largest lag */
     int32_t walk_helped;
     int32_t walk_lag;
     /* vertices whose final position is not finite (NaN or infinite).  The reference's move
@@ -122,7 +124,7 @@ typedef struct {
 } bl_stats;
 
 /* The batch walks of the Barnes-Hut family (bl_stats.walk); the library picks one per graph and
- * batch size (tests may force one with the environment variable BL_WALK=grid|single|pf|row) */
+ * batch size (tests may force one with the environment variable BL_WALK=grid|wide|single|pf|row) */
 #define BL_WALK_NONE 0
 #define BL_WALK_ROW_ORDER 1  /* FP64 mode: the reference's row order, one workgroup */
 #define BL_WALK_PF 2         /* one workgroup, a batch's CSR entries split evenly over its threads */
@@ -134,6 +136,10 @@ typedef struct {
                                 (D = 20 by default; chosen when B <= 256, every row has at most 64
                                 entries and no vertex has more than 4 neighbours in the D - 1
                                 batches before its own) */
+#define BL_WALK_WIDE 7       /* one walking workgroup that sums only the neighbours moved in the
+                                last D_b - 1 batches (D_b <= 16 per batch, from an LDS ring) and
+                                helper workgroups that sum the others in chunks of rows; chosen
+                                for B <= 1024 where batches hold many CSR entries (long rows) */
 
 /* Fills *p with the reference CLI defaults. */
 int bl_params_default(bl_params *p);

@@ -223,6 +223,60 @@ def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo, weight
         start = out["single"]
 
 
+@pytest.mark.parametrize("kind,algo,weighted,batch", [("hubs", 4, False, 1024), ("hubs", 40, False, 1000),
+                                                      ("hubs", 4, True, 256), ("rmat", 4, True, 1024),
+                                                      ("rmat", 4, False, 512), ("dense", 4, True, 1024)])
+def test_wide_walk_matches_single(bl, monkeypatch, kind, algo, weighted, batch):
+    """The wide lagged walk (a walking workgroup sums the neighbours moved in the last D_b - 1
+    batches from an LDS ring, helper workgroups the others in chunks of rows) agrees with the
+    one-workgroup walk up to the order of partial sums, at iteration 1 and at iteration 2
+    teacher-forced from the one-workgroup walk's iteration 1; its layout is the same bits
+    whatever the number of helpers (one helper makes the walker wait at almost every batch),
+    and a smaller largest lag moves entries between the walker and the helpers without changing
+    the result beyond the order of sums.  R-MAT and the last partial batches (21024 vertices)
+    cover hubs in the first batches and the threads past a batch's end."""
+    if kind == "hubs":
+        rp, ci = _hubby_graph(12000, 4, 3)
+    elif kind == "rmat":
+        n0, e0, _ = graphs.rmat(15, 8, seed=2)
+        rp, ci = graphs.csr_from_edges(n0, e0)
+    else:
+        rng = np.random.default_rng(5)
+        e = rng.integers(0, 12000, size=(12000 * 30, 2))
+        rp, ci = graphs.csr_from_edges(12000, e[e[:, 0] != e[:, 1]])
+    n = len(rp) - 1
+    w = np.random.default_rng(4).uniform(0.5, 2.0, len(ci)).astype(np.float32) if weighted else None
+    x0 = np.random.default_rng(2).random((n, 2))
+    print(f"[case] wide walk {kind} algo {algo} weighted {weighted} batch {batch} n {n} nnz {len(ci)}", flush=True)
+    runs = (("single", dict(BL_WALK="single")), ("wide", dict(BL_WALK="wide")),
+            ("wide_h1", dict(BL_WALK="wide", BL_WIDE_H="1")), ("wide_h3", dict(BL_WALK="wide", BL_WIDE_H="3")),
+            ("wide_d5", dict(BL_WALK="wide", BL_WIDE_D="5")))
+    start = x0
+    for it in range(2):
+        out = {}
+        for mode, env in runs:
+            for k in ("BL_WALK", "BL_WIDE_H", "BL_WIDE_D"):
+                monkeypatch.delenv(k, raising=False)
+            for k, v in env.items():
+                monkeypatch.setenv(k, v)
+            with bl.Graph(rp, ci, w) as g:
+                out[mode], st = g.run(start, algo=algo, iters=1, batch=batch, bht=1.2, step0=0.999 ** it)
+            out[mode + "_E"] = st.energy
+            if mode != "single":
+                assert st.walk == bl.WALK_WIDE, (mode, st.walk)
+                assert st.walk_lag == (5 if mode == "wide_d5" else 16), (mode, st.walk_lag)
+                print(f"  it {it} {mode}: batches ready when reached {st.walk_helped} of {(n + batch - 1) // batch}")
+        d = diameter(out["single"])
+        for mode in ("wide", "wide_d5"):
+            err = np.abs(out[mode] - out["single"]).max() / d
+            print(f"  it {it} {mode}: {err:.2e} of the diameter")
+            assert err <= 1e-10, (it, mode)
+            assert abs(out[mode + "_E"] - out["single_E"]) <= 1e-10 * out["single_E"], (it, mode)
+        for mode in ("wide_h1", "wide_h3"):
+            assert np.array_equal(out[mode], out["wide"]) and out[mode + "_E"] == out["wide_E"], (it, mode)
+        start = out["single"]
+
+
 @pytest.mark.parametrize("kind,weighted,batch,algo,prec", [("rgg", False, 256, 4, 0), ("rgg", True, 256, 4, 0),
                                                            ("rgg", False, 100, 4, 0), ("rgg", True, 37, 4, 0),
                                                            ("rgg", False, 256, 40, 0), ("rgg", False, 256, 4, 2),

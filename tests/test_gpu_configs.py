@@ -40,20 +40,28 @@ def rmat15():
     return rp, ci, wc
 
 
-def test_rmat_algo4_multi_workgroup_walk_vs_oracle(bl, oracle, rmat15):
+@pytest.mark.parametrize("walk", ["wide", "grid"])
+def test_rmat_algo4_walks_vs_oracle(bl, oracle, rmat15, monkeypatch, walk):
+    """R-MAT, FP32 weights, -batch 1024 -bht 1.2: the default walk (the wide lagged walk) and
+    the multi-workgroup walk (BL_WALK=grid) against the oracle, iteration 1 and a teacher-forced
+    iteration 2; the FP64 mode bit-exact."""
     rp, ci, w = rmat15
     n = len(rp) - 1
     x0 = graphs.uniform_init(n, 0)
-    print(f"[case] rmat15 algo 4 n {n} nnz {len(ci)} batch 1024", flush=True)
+    print(f"[case] rmat15 algo 4 n {n} nnz {len(ci)} batch 1024 walk {walk}", flush=True)
     ref1, e1 = oracle.bh_layout(rp, ci, x0, 1, 1024, theta=1.2, w_edge=w)
     ref2, e2 = oracle.bh_layout(rp, ci, ref1, 1, 1024, theta=1.2, step0=0.999, w_edge=w)
     kw = dict(algo=4, iters=1, batch=1024, bht=1.2)
+    monkeypatch.delenv("BL_WALK", raising=False)
+    if walk == "grid":
+        monkeypatch.setenv("BL_WALK", "grid")
     with bl.Graph(rp, ci, w) as g:
         x64, s64 = g.run(x0, precision=bl.PREC_FP64, **kw)
         y64, t64 = g.run(ref1, precision=bl.PREC_FP64, step0=0.999, **kw)
         x1, s1 = g.run(x0, **kw)
         x2, s2 = g.run(ref1, step0=0.999, **kw)  # teacher-forced from the oracle's iteration 1
-    assert s1.walk == bl.WALK_GRID and s2.walk == bl.WALK_GRID
+    want = bl.WALK_WIDE if walk == "wide" else bl.WALK_GRID
+    assert s1.walk == want and s2.walk == want
     assert np.array_equal(x64, ref1) and s64.energy == e1
     assert np.array_equal(y64, ref2) and t64.energy == e2
     err1 = np.abs(x1 - ref1).max() / diameter(ref1)
@@ -85,20 +93,24 @@ def test_rmat_algo8_batch1024_vs_oracle(bl, oracle, rmat15):
 
 
 @pytest.mark.timeout(900)
-def test_rmat20_tree_and_iteration(bl, oracle):
+def test_rmat20_tree_and_iteration(bl, oracle, monkeypatch):
     n, e, w = graphs.rmat(20, 8, seed=2)
     rp, ci, wc = graphs.csr_from_edges(n, e, w)
     x0 = graphs.uniform_init(n, 0)
-    print(f"[case] rmat20 n {n} nnz {len(ci)}: one iteration, FP64 and default", flush=True)
+    print(f"[case] rmat20 n {n} nnz {len(ci)}: one iteration, FP64, default (wide walk) and grid walk", flush=True)
     kw = dict(algo=4, iters=1, batch=1024, bht=1.2)
+    monkeypatch.delenv("BL_WALK", raising=False)
     with bl.Graph(rp, ci, wc) as g:
         x64, s64 = g.run(x0, precision=bl.PREC_FP64, **kw)
         x1, s1 = g.run(x0, **kw)
-    assert s1.walk == bl.WALK_GRID
-    err = np.abs(x1 - x64).max() / diameter(x64)
-    print(f"rmat20: default vs FP64 after one iteration {err:.2e} of the diameter, energy "
-          f"{abs(s1.energy - s64.energy) / s64.energy:.2e}", flush=True)
-    assert err <= 1e-4 and abs(s1.energy - s64.energy) <= 1e-4 * s64.energy
+        monkeypatch.setenv("BL_WALK", "grid")
+        xg, sg = g.run(x0, **kw)
+    assert s1.walk == bl.WALK_WIDE and sg.walk == bl.WALK_GRID
+    for name, x, st in (("wide", x1, s1), ("grid", xg, sg)):
+        err = np.abs(x - x64).max() / diameter(x64)
+        print(f"rmat20: {name} walk vs FP64 after one iteration {err:.2e} of the diameter, energy "
+              f"{abs(st.energy - s64.energy) / s64.energy:.2e}", flush=True)
+        assert err <= 1e-4 and abs(st.energy - s64.energy) <= 1e-4 * s64.energy
     # the tree of the layout's own (clustered) positions
     print("[case] rmat20 Morton scales, permutation and tree of the iteration-1 positions", flush=True)
     xs, ys, mm = bl.bh_morton(x64)
