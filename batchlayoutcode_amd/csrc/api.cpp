@@ -177,7 +177,7 @@ extern "C" int bl_graph_destroy(bl_graph *g) {
                      &g->pos, &g->part, &g->eterm, &g->energy, &g->scratch, &g->guard, &g->far_model, &g->cpre, &g->cpv,
                      &g->early_rp, &g->early_ci, &g->early_w, &g->late_rp, &g->late_ci, &g->late_w,
                      &g->grid_rows, &g->lag_erp, &g->lag_esp, &g->lag_eci, &g->lag_ew, &g->lag_rofs, &g->lag_rw,
-                     &g->lag_pre, &g->lag_ready, &g->wide_near_rp, &g->wide_near_code, &g->wide_near_w,
+                     &g->lag_pre, &g->lag_ready, &g->lag_ring_of, &g->lag_gstart, &g->lag_sink, &g->wide_near_rp, &g->wide_near_code, &g->wide_near_w,
                      &g->wide_npa, &g->wide_npb, &g->wide_ring_of, &g->wide_far_rp, &g->wide_far_ci, &g->wide_far_w,
                      &g->wide_chunks, &g->wide_cfr, &g->wide_nch, &g->wide_pre, &g->wide_cnt, &g->wide_sink};
     for (DevBuf *b : all) b->release();

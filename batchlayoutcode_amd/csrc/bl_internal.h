@@ -75,14 +75,16 @@ struct bl_graph {
     // rows of the workgroup's share
     int32_t grid_rows_batch = 0;
     bl::DevBuf grid_rows;
-    // the lagged walk (walk_lag.hip) at batch size lag_batch and lag lag_d: each vertex's early
-    // entries split into settled (at least lag_d batches back) and recent ones, the recent ones'
-    // ring offsets and weights (4 per vertex), the helpers' per-vertex sums and per-batch flags
-    // (launch generations)
-    int32_t lag_batch = 0, lag_d = 0, lag_gen = 0;
+    // the lagged walk (walk_lag.hip) at batch size lag_batch and lag lag_d (groups of lag_width
+    // vertices at most, lag_ng of them, starting at lag_gstart): each vertex's early entries
+    // split into settled (at least lag_d groups back) and recent ones, the recent ones' ring slots
+    // and weights (4 per vertex), each vertex's own ring slot (lag_rcap slots), the helpers'
+    // per-vertex sums and per-group flags (launch generations)
+    int32_t lag_batch = 0, lag_d = 0, lag_gen = 0, lag_width = 0, lag_ng = 0, lag_rcap = 0;
     int32_t lag_auto_batch = 0, lag_auto_d = 0;  // the default lag for batch size lag_auto_batch
     bool lag_ok = false;
-    bl::DevBuf lag_erp, lag_esp, lag_eci, lag_ew, lag_rofs, lag_rw, lag_pre, lag_ready;
+    bl::DevBuf lag_erp, lag_esp, lag_eci, lag_ew, lag_rofs, lag_rw, lag_pre, lag_ready, lag_ring_of, lag_gstart,
+        lag_sink;
     // the wide lagged walk (walk_wide.hip) at batch size wide_batch and largest lag wide_dmax:
     // each row's entries split into near ones (the walker's, with ring slots) and far ones (the
     // helpers', in chunks of whole rows), the walker's per-thread tables, the helpers' sums and

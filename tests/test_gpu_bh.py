@@ -223,6 +223,31 @@ def test_multi_workgroup_walk_matches_single(bl, monkeypatch, kind, algo, weight
         start = out["single"]
 
 
+def _lag_steps(rp, ci, B):
+    """The lagged walk's steps for batches of B: consecutive batches grouped while none of a
+    joining batch's vertices has a neighbour in the group (at most 512 // B batches), when that
+    makes at most four fifths as many steps as batches; else one batch a step."""
+    n = len(rp) - 1
+    nb = (n + B - 1) // B
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    bi, bj = rows // B, ci // B
+    early = bj < bi
+    last_early = np.full(nb, -1)  # the largest neighbour id of a batch's vertices below the batch
+    np.maximum.at(last_early, bi[early], ci[early])
+
+    def steps(T):
+        gmax, ng, cur0, size = max(1, T // B), 1, 0, 1
+        for b in range(1, nb):
+            if size < gmax and last_early[b] < cur0:
+                size += 1
+            else:
+                ng, cur0, size = ng + 1, b * B, 1
+        return ng
+
+    g512 = steps(512)
+    return g512 if 5 * g512 <= 4 * nb else steps(256)
+
+
 @pytest.mark.parametrize("kind,algo,weighted,batch", [("hubs", 4, False, 1024), ("hubs", 40, False, 1000),
                                                       ("hubs", 4, True, 256), ("rmat", 4, True, 1024),
                                                       ("rmat", 4, False, 512), ("dense", 4, True, 1024)])
@@ -316,7 +341,8 @@ def test_row_walk_matches_entry_split_walk(bl, monkeypatch, kind, weighted, batc
 
 @pytest.mark.parametrize("kind,weighted,batch,algo", [("rgg", False, 256, 4), ("rgg", True, 256, 4), ("rgg", False, 100, 4),
                                                       ("grid", False, 256, 4), ("delaunay", False, 256, 40),
-                                                      ("delaunay", True, 200, 4)])
+                                                      ("delaunay", True, 200, 4), ("delaunay", False, 64, 4),
+                                                      ("delaunay", True, 100, 40)])
 def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, weighted, batch, algo):
     """The lagged walk (helper workgroups sum the neighbours moved at least D batches earlier; the
     walker sums the rest and forms a batch's sum itself when its helper is late) against the
@@ -326,7 +352,8 @@ def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, w
     refined reciprocal square root instead of an IEEE square root and division, so the two agree
     to rounding: 1e-10 of the diameter after the first iteration, and after a third iteration
     teacher-forced from the row walk's second (free-running, such differences grow chaotically:
-    2e-9 after three iterations)."""
+    2e-9 after three iterations).  At batches of 64 and 100 on the Delaunay graph most
+    consecutive batches share no edge, and the walk moves groups of up to 512 vertices a step."""
     if kind == "grid":  # natural order: every early neighbour is in the previous batch
         rp, ci = graphs.csr_from_edges(*graphs.grid(160, 160))
     elif kind == "delaunay":
@@ -357,7 +384,10 @@ def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, w
                     x2, _ = g.run(x0, algo=algo, iters=2, batch=batch, bht=1.2)
                 out[mode + "_3tf"], _ = g.run(x2, algo=algo, iters=1, batch=batch, bht=1.2, step0=0.999 ** 2)
     nb = (n + batch - 1) // batch
-    print(f"batches {nb}, formed by the helpers (last iteration): {helped}")
+    ng = _lag_steps(rp, ci, batch)
+    print(f"batches {nb}, walk steps (groups) {ng}, formed by the helpers (last iteration): {helped}")
+    if batch <= 100 and kind == "delaunay":
+        assert ng < nb // 2  # groups of several batches
     d = diameter(out["row"])
     for m in ("lag", "lag_d12"):
         e1 = np.abs(out[m + "_1"] - out["row_1"]).max() / d
@@ -367,7 +397,7 @@ def test_lag_walk_matches_row_walk_and_is_deterministic(bl, monkeypatch, kind, w
         assert e1 <= 1e-10 and e3 <= 1e-10, m
     assert abs(out["lag_E"] - out["row_E"]) <= 1e-6 * out["row_E"]
     assert helped["lag_h0"] == 0 and helped["lag_d12_h0"] == 0
-    assert helped["lag"] > nb // 2 and helped["lag_d12"] > 0
+    assert helped["lag"] > ng // 2 and helped["lag_d12"] > 0
     assert np.array_equal(out["lag"], out["lag_h0"]) and out["lag_E"] == out["lag_h0_E"]
     assert np.array_equal(out["lag"], out["lag_h4"]) and out["lag_E"] == out["lag_h4_E"]
     assert np.array_equal(out["lag_d12"], out["lag_d12_h0"]) and out["lag_d12_E"] == out["lag_d12_h0_E"]
@@ -391,7 +421,8 @@ def test_lag_walk_declines_crowded_recent_batches(bl, monkeypatch):
     r_s, g_s = r_s[keep], g_s[keep]
     first = np.searchsorted(r_s, np.arange(n))
     cnt = np.bincount(r_s, minlength=n)
-    d_fit = min(27, ((n + B - 1) // B) // 2, int(g_s[first[cnt >= 5] + 4].min()) if (cnt >= 5).any() else 27)
+    assert _lag_steps(rp, ci, B) == (n + B - 1) // B  # every consecutive pair shares an edge: no groups
+    d_fit = min(48, ((n + B - 1) // B) // 2, int(g_s[first[cnt >= 5] + 4].min()) if (cnt >= 5).any() else 48)
     for k in ("BL_WALK", "BL_LAG_H", "BL_LAG_D"):
         monkeypatch.delenv(k, raising=False)
     outs = {}
