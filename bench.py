@@ -17,8 +17,9 @@ with the physically consistent repulsion sum_j (x_j - x_i)/|x_j - x_i|^2 (algo 2
 
 Qualification: for each mode, theta is swept from large (fast) to small; a theta qualifies when,
 after 10 iterations from each of `--starts` uniform starts (seeds 0, 1, ...), the mean over the
-starts of the signed relative difference of the final energy to the exact layout's from the same
-start is within 1%.  The fastest (largest) qualifying theta is kept.
+starts of the absolute relative difference of the final energy to the exact layout's from the
+same start is within 1% (differences of opposite sign do not cancel).  The fastest (largest)
+qualifying theta is kept.
 
 A "step" is one layout iteration (every vertex updated once).  `value` = learned vertex-updates/s
 = n * steps / seconds of the timed region at the learned mode's qualifying theta, positions
@@ -170,7 +171,7 @@ def c4_leg(args, torch, api, graphs):
                     es.append(st.energy)
                     secs += st.seconds
                 per = [(a - b) / b for a, b in zip(es, e_exact)]
-                r = dict(theta=th, energy_rel_diff_per_start=per, energy_rel_diff_vs_exact=abs(float(np.mean(per))),
+                r = dict(theta=th, energy_rel_diff_per_start=per, energy_rel_diff_vs_exact=float(np.mean(np.abs(per))),
                          vertex_updates_per_s=n * QUAL_ITERS * len(starts) / secs)
                 sweep.append(r)
                 log(f"{name} theta {th}: {r['vertex_updates_per_s']:.3e} v-u/s, E vs exact "
@@ -240,8 +241,9 @@ def c4_roofline(n, nnz, lp):
                      "16 B repulsion + 16 B written position + 8 B energy term per vertex, 4 B column id + 16 B "
                      "neighbour position per CSR entry) / mean time of the walk and its pre-pass; traffic: their "
                      "HBM bytes per iteration from FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size "
-                     "(profiles/traffic_k_walk_lag_delaunay2m.json); the walk is bound by its per-batch latency "
-                     "(one workgroup walks the 6,497 sequential batches), not by bytes")
+                     "(profiles/traffic_k_walk_lag_delaunay2m.json); the walk is bound by its per-step latency "
+                     "(one workgroup walks the batches in sequence, in groups of consecutive batches with no "
+                     "edge between them), not by bytes")
 
 
 def algo2_leg(args, torch, api, graphs):
