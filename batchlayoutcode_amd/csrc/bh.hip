@@ -8,11 +8,12 @@
 // Per iteration, from the positions at the start of the iteration (:1041-1047):
 //   1. k_frame_*   min x, min y, MAXMIN = max(dy, dx)                       (FP64, exact)
 //   2. k_keys      64-bit key = interleave(yscale, xscale), y bit above x bit
-//   3. radix sort  LSD, 8 passes of 8 bits over (key, vertex id), stable
-//   4. k_gather    positions in Morton order
+//   3. radix sort  LSD, 8 passes of 8 bits over (key, vertex id), stable: per-tile digit
+//                  counts, a scan per digit, a scatter staged through LDS
+//   4. gather      positions in Morton order (the last scatter pass writes them)
 //   5. tree        node = cell where two adjacent keys first differ (base-4 digit d of
-//                  k_i ^ k_i+1); its range, quadrants and children come from binary
-//                  searches on the sorted keys -- the reference's tree with single-child
+//                  k_i ^ k_i+1); its range, quadrants and children come from galloping and
+//                  binary searches on the sorted keys -- the reference's tree with single-child
 //                  chains collapsed into their lowest node (a chain shares one centroid,
 //                  and the smallest diameter in the chain decides acceptance, so the
 //                  traversal result is the same)
@@ -50,15 +51,11 @@ int launch_energy(const double *eterm, int n, double *out, int precision, hipStr
 namespace {
 
 constexpr int WG = 256;
-// Keys per thread per sort tile (tile = WG x rounds keys): at least 4, and enough that the
-// 256 x tiles histogram counts fit the one-launch scan (32,768 counts) up to 1M keys;
-// beyond that 16 (the three-launch scan).  Smaller tiles give more workgroups and fewer
-// serial rounds each (scatter pass on 250k keys: 16 rounds, 62 workgroups, 16 us; 8 rounds,
-// 122 workgroups, 10 us; 4 rounds 7.4 us, but 62k counts then need the three-launch scan).
-inline int sort_rounds(int n) {
-    const int r = (n + 32767) / 32768;
-    return r <= 32 ? std::max(4, r) : 16;
-}
+// Keys per thread per sort tile (tile = WG x SORT_ROUNDS keys, staged in LDS by the scatter:
+// 30 KB).  The per-digit row scan takes any number of tiles.  On 1.66M keys a pass took
+// 14.6 us with 8 rounds and with 16, against 27.5 us unstaged with 16 rounds and 34 us
+// unstaged with 4.
+constexpr int SORT_ROUNDS = 8;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = WG * SCAN_ITEMS;     // 2048 entries per workgroup
 constexpr int STACK = 128;                     // per-wave traversal stack (depth <= 32 -> <= 100 entries)
@@ -307,28 +304,91 @@ __global__ __launch_bounds__(WG) void k_sort_hist(const uint64_t *__restrict__ k
     hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: items are ranked in tile order (round r, then thread index), using wave
-// ballots to find the lanes that share a digit and per-wave counts in LDS.
+// Per-digit scan of the histogram: workgroup d turns row d (the blocks' counts of digit d) into
+// its exclusive prefix in place and writes the row's total to dtot[d].  Thread t takes 8
+// consecutive counts of each chunk of WG x 8.
+__global__ __launch_bounds__(WG) void k_sort_rowscan(int *__restrict__ hist, int nblk, int *__restrict__ dtot) {
+    __shared__ int wsum[WG / 64];
+    int *row = hist + (size_t)blockIdx.x * nblk;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int carry = 0;
+    for (int c0 = 0; c0 < nblk; c0 += WG * 8) {
+        const int i0 = c0 + 8 * t;
+        int v[8], a = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = i0 + k < nblk ? row[i0 + k] : 0;
+            a += v[k];
+        }
+        int x = a;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        int run = carry + x - a, tot = 0;
+#pragma unroll
+        for (int p = 0; p < WG / 64; ++p) {
+            const int sp = wsum[p];
+            tot += sp;
+            run += p < w ? sp : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (i0 + k < nblk) row[i0 + k] = run;
+            run += v[k];
+        }
+        carry += tot;
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
+    if (t == 0) dtot[blockIdx.x] = carry;
+}
+
+// Stable scatter of one tile of WG x R keys, staged through LDS so that the global writes of a
+// digit's run are consecutive.  Wave w holds tile entries w*64R + 64r + lane (r < R), so the
+// waves' entries, and each wave's rounds, follow the input order: a wave ranks its keys by
+// ballots against its own running digit counts in LDS, with no barrier between rounds.  The
+// tile's digit starts (the scan of the waves' counts) place every key in LDS in sorted order;
+// thread t then writes entries t, t + WG, ... to the digit's global bucket, which starts at the
+// keys of all smaller digits (the exclusive scan of dtot) plus the tile's row prefix.  GATHER
+// (the last pass): also spos[g] = pos[id], the positions in sorted order.
+template <int R, bool GATHER>
 __global__ __launch_bounds__(WG) void k_sort_scatter(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, int *__restrict__ vout, int n,
-                                                     int shift, int nblk, int rounds, const int *__restrict__ offs) {
-    __shared__ int running[256];
-    __shared__ int wcount[WG / 64][256];
-    __shared__ int base_off[256];
+                                                     int shift, int nblk, const int *__restrict__ offs,
+                                                     const int *__restrict__ dtot, const double2 *__restrict__ pos,
+                                                     double2 *__restrict__ spos) {
+    constexpr int NW = WG / 64, TILE = WG * R;
+    __shared__ uint64_t sk[TILE];
+    __shared__ int sv[TILE];
+    __shared__ int wrun[NW][256];  // each wave's running digit counts, then its LDS base per digit
+    __shared__ int lstart[256];    // start of digit d in the staged tile
+    __shared__ int gstart[256];    // global start of the tile's run of digit d
+    __shared__ int wtot[2][NW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    running[tid] = 0;
-    base_off[tid] = offs[tid * nblk + blockIdx.x];
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int base = blockIdx.x * WG * rounds;
-    for (int r = 0; r < rounds; ++r) {
+    const int base = blockIdx.x * TILE;
+    const int cnt = min(TILE, n - base);
+    const int cd = dtot[tid], row = offs[tid * nblk + blockIdx.x];
+    const int wb = w * 64 * R + lane;
+    uint64_t k[R];
+    int v[R];
 #pragma unroll
-        for (int q = 0; q < WG / 64; ++q) wcount[q][tid] = 0;
-        __syncthreads();
-        const int i = base + r * WG + tid;
-        const bool valid = i < n;
-        const uint64_t k = valid ? kin[i] : 0ull;
-        const int v = valid ? vin[i] : 0;
-        const int d = (int)((k >> shift) & 255u);
+    for (int r = 0; r < R; ++r) {
+        const int e = wb + 64 * r;
+        k[r] = e < cnt ? kin[base + e] : 0ull;
+        v[r] = e < cnt ? vin[base + e] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < NW; ++q) wrun[q][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int rk[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool valid = wb + 64 * r < cnt;
+        const int d = (int)((k[r] >> shift) & 255u);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -336,29 +396,73 @@ __global__ __launch_bounds__(WG) void k_sort_scatter(const uint64_t *__restrict_
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const int rank = __popcll(peers & lt);
-        if (valid && rank == 0) wcount[w][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pos = base_off[d] + running[d] + rank;
-            for (int q = 0; q < w; ++q) pos += wcount[q][d];
-            kout[pos] = k;
-            vout[pos] = v;
+        const int below = __popcll(peers & lt);
+        // every lane reads before the leaders write: one wave's LDS operations complete in order
+        const int before = wrun[w][d];
+        rk[r] = before + below;
+        if (valid && below == 0) wrun[w][d] = before + __popcll(peers);
+    }
+    __syncthreads();
+    {
+        // thread tid = digit: the tile's count and its start in LDS; the digit's global start
+        int c[NW], tot = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            c[q] = wrun[q][tid];
+            tot += c[q];
+        }
+        int x = tot, y = cd;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int xs = __shfl_up(x, off, 64), ys = __shfl_up(y, off, 64);
+            if (lane >= off) {
+                x += xs;
+                y += ys;
+            }
+        }
+        if (lane == 63) {
+            wtot[0][w] = x;
+            wtot[1][w] = y;
         }
         __syncthreads();
-        int add = 0;
+        int ls = x - tot, gs = y - cd;
 #pragma unroll
-        for (int q = 0; q < WG / 64; ++q) add += wcount[q][tid];
-        running[tid] += add;
+        for (int p = 0; p < NW; ++p) {
+            ls += p < w ? wtot[0][p] : 0;
+            gs += p < w ? wtot[1][p] : 0;
+        }
+        lstart[tid] = ls;
+        gstart[tid] = gs + row;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            wrun[q][tid] = ls;
+            ls += c[q];
+        }
     }
-}
-
-// ---------------------------------------------------------------------------------------
-// 4. gather positions into Morton order
-__global__ __launch_bounds__(WG) void k_gather(const double2 *__restrict__ pos, const int *__restrict__ perm, int n,
-                                               double2 *__restrict__ spos) {
-    const int k = blockIdx.x * WG + threadIdx.x;
-    if (k < n) spos[k] = pos[perm[k]];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (wb + 64 * r < cnt) {
+            const int d = (int)((k[r] >> shift) & 255u);
+            const int l = wrun[w][d] + rk[r];
+            sk[l] = k[r];
+            sv[l] = v[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = r * WG + tid;
+        if (e < cnt) {
+            const uint64_t key = sk[e];
+            const int d = (int)((key >> shift) & 255u);
+            const int g = gstart[d] + e - lstart[d];
+            const int id = sv[e];
+            kout[g] = key;
+            vout[g] = id;
+            if (GATHER) spos[g] = pos[id];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -381,6 +485,28 @@ __device__ __forceinline__ int last_with_prefix(const uint64_t *keys, int lo, in
         if (prefix_of(keys[m], d) <= pfx) lo = m; else hi = m;
     }
     return lo;
+}
+// first index in [0, i] whose 2d-bit prefix equals pfx, given keys[i] has it: probes leftwards
+// at distances 1, 3, 7, ... from i (a node's points are contiguous and most nodes are small),
+// then a binary search over the last interval
+__device__ __forceinline__ int gallop_first(const uint64_t *keys, int i, int d, uint64_t pfx) {
+    int hi = i, step = 1, lo = i - 1;
+    while (lo >= 0 && prefix_of(keys[lo], d) == pfx) {
+        hi = lo;
+        step <<= 1;
+        lo = hi - step;
+    }
+    return first_with_prefix(keys, lo < 0 ? 0 : lo + 1, hi, d, pfx);
+}
+// last index in [j, n) whose prefix equals pfx, given keys[j] has it (the mirror image)
+__device__ __forceinline__ int gallop_last(const uint64_t *keys, int j, int n, int d, uint64_t pfx) {
+    int lo = j, step = 1, hi = j + 1;
+    while (hi < n && prefix_of(keys[hi], d) == pfx) {
+        lo = hi;
+        step <<= 1;
+        hi = lo + step;
+    }
+    return last_with_prefix(keys, lo, hi < n ? hi : n, d, pfx);
 }
 // first index in [lo, hi) whose digit d is >= q (lower_bound of BarnesHut.h:143-146)
 __device__ __forceinline__ int lower_digit(const uint64_t *keys, int lo, int hi, int d, int q) {
@@ -412,7 +538,7 @@ __global__ __launch_bounds__(WG) void k_tree_bounds(const uint64_t *__restrict__
     }
     const int d = __clzll((long long)(a ^ b)) >> 1;
     const uint64_t pfx = prefix_of(a, d);
-    const int L = d == 0 ? 0 : first_with_prefix(keys, 0, i + 1, d, pfx);
+    const int L = d == 0 ? 0 : gallop_first(keys, i, d, pfx);
     isrep[i] = digit_at(keys[L], d) == digit_at(a, d) ? 1 : 0;
 }
 
@@ -436,8 +562,8 @@ __global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ 
     const uint64_t a = keys[i];
     const int d = __clzll((long long)(a ^ keys[i + 1])) >> 1;
     const uint64_t pfx = prefix_of(a, d);
-    const int L = d == 0 ? 0 : first_with_prefix(keys, 0, i + 1, d, pfx);
-    const int R = d == 0 ? n - 1 : last_with_prefix(keys, i + 1, n, d, pfx);
+    const int L = d == 0 ? 0 : gallop_first(keys, i, d, pfx);
+    const int R = d == 0 ? n - 1 : gallop_last(keys, i + 1, n, d, pfx);
     int bnd[5];
     bnd[0] = L;
     bnd[4] = R + 1;
@@ -1807,7 +1933,7 @@ struct TreeBufs {
 
 // Exclusive scan of in[0, n) into out, the total at bsum[nb].  `in` must hold at least
 // SCAN_SINGLE_MAX ints whatever n is (k_scan_single reads all its tiles; ensure_bh sizes the
-// histogram and boundary-flag buffers so).
+// boundary-flag buffer so).
 int scan_excl(const int *in, int n, int *out, int *bsum, hipStream_t s) {
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (n <= SCAN_SINGLE_MAX) {  // one launch instead of three
@@ -1825,13 +1951,11 @@ int scan_excl(const int *in, int n, int *out, int *bsum, hipStream_t s) {
 int ensure_bh(BHWork &w, int n) {
     if (w.n_alloc >= n) return BL_OK;
     const size_t N = (size_t)n;
-    // every size up to n: the tiles are never smaller than WG x 4 keys
-    const int nblk = (n + WG * 4 - 1) / (WG * 4);
-    const int nscan = std::max(256 * nblk, n);
-    const int nsb = (nscan + SCAN_TILE - 1) / SCAN_TILE + 1;
+    // every size up to n: the tiles are never smaller than WG x SORT_ROUNDS keys
+    const int nblk = (n + WG * SORT_ROUNDS - 1) / (WG * SORT_ROUNDS);
+    const int nsb = (n + SCAN_TILE - 1) / SCAN_TILE + 1;
     if (w.keys_a.ensure(8 * N) || w.keys_b.ensure(8 * N) || w.idx_a.ensure(4 * N) || w.idx_b.ensure(4 * N) ||
-        w.hist.ensure(4 * std::max((size_t)256 * nblk, (size_t)SCAN_SINGLE_MAX)) ||
-        w.scan_tmp.ensure(4 * (size_t)(256 * nblk + 1)) || w.spos.ensure(16 * N) ||
+        w.hist.ensure(4 * (size_t)256 * nblk) || w.scan_tmp.ensure(4 * 256) || w.spos.ensure(16 * N) ||
         w.frame.ensure(sizeof(double4) * 1024 + 64) || w.isrep.ensure(4 * std::max(N, (size_t)SCAN_SINGLE_MAX)) ||
         w.nodeid.ensure(4 * N) || w.node_child.ensure(sizeof(BHNode) * N) || w.rep.ensure(16 * N) ||
         w.misc.ensure(4 * (size_t)(64 + nsb)) || w.node_lo.ensure(4 * N) || w.node_cnt.ensure(4 * N))
@@ -1858,23 +1982,26 @@ int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *
     int *ia = w.idx_a.as<int>(), *ib = w.idx_b.as<int>();
     const int g1 = (n + WG - 1) / WG;
     hipLaunchKernelGGL(k_keys, dim3(g1), dim3(WG), 0, s, d_pos, n, frame, ka, ia, d_xs, d_ys);
-    const int rounds = sort_rounds(n);
+    const int rounds = SORT_ROUNDS;
     const int nblk = (n + WG * rounds - 1) / (WG * rounds);
+    double2 *spos = w.spos.as<double2>();
     int *hist = w.hist.as<int>();
-    int *offs = w.scan_tmp.as<int>();
+    int *dtot = w.scan_tmp.as<int>();
     int *bsum = misc + 64;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(WG), 0, s, ka, n, shift, nblk, rounds, hist);
-        rc = scan_excl(hist, 256 * nblk, offs, bsum, s);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblk), dim3(WG), 0, s, ka, ia, kb, ib, n, shift, nblk, rounds, offs);
+        hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(WG), 0, s, hist, nblk, dtot);
+        if (pass < 7)
+            hipLaunchKernelGGL((k_sort_scatter<SORT_ROUNDS, false>), dim3(nblk), dim3(WG), 0, s, ka, ia, kb, ib, n,
+                               shift, nblk, hist, dtot, d_pos, spos);
+        else
+            hipLaunchKernelGGL((k_sort_scatter<SORT_ROUNDS, true>), dim3(nblk), dim3(WG), 0, s, ka, ia, kb, ib, n,
+                               shift, nblk, hist, dtot, d_pos, spos);
         std::swap(ka, kb);
         std::swap(ia, ib);
     }
-    // after 8 passes the sorted data is back in keys_a / idx_a
-    double2 *spos = w.spos.as<double2>();
-    hipLaunchKernelGGL(k_gather, dim3(g1), dim3(WG), 0, s, d_pos, ia, n, spos);
+    // after 8 passes the sorted data is back in keys_a / idx_a, the positions in spos
     if (n >= 2) {
         int *isrep = w.isrep.as<int>(), *nodeid = w.nodeid.as<int>();
         BHNode *nodes = w.node_child.as<BHNode>();

@@ -66,11 +66,13 @@ def test_duplicate_points_are_an_error(bl):
     assert ei.value.code == -5
 
 
-@pytest.mark.parametrize("n", [131072, 131073, 1048576, 1048577])
+@pytest.mark.parametrize("n", [1023, 1025, 32769, 32770, 2097152, 2097153])
 def test_sort_tile_and_scan_boundaries(bl, oracle, n):
-    # the radix sort's tile size follows n, and its histogram counts go to the one-workgroup
-    # scan up to 32,768 counts (n = 131072 and 1048576 land exactly there) and to the
-    # three-launch scan beyond (n = 1048577): permutation and FP64 forces stay bit-exact
+    # the radix sort's tiles hold 1,024 keys (n = 1023, 1025: a partial tile); the node-id
+    # scan over n - 1 boundary flags takes one workgroup up to 32,768 flags and three
+    # launches beyond (n = 32769, 32770); the per-digit row scan works in chunks of 2,048
+    # tiles (n = 2097152 fills one exactly, 2097153 starts a second): permutation and FP64
+    # forces stay bit-exact
     pts = np.random.default_rng(n).random((n, 2))
     print(f"[case] sort boundaries n {n}", flush=True)
     t = bl.bh_tree(pts)
