@@ -44,7 +44,7 @@ void DevBuf::release() {
 void BHWork::release() {
     DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &hist, &spos, &frame, &isrep, &nodeid,
                      &node_child, &node_cnt, &node_lo, &scan_tmp,
-                     &rep, &misc, &far_cells, &far_arrive, &judge, &gwalk};
+                     &rep, &misc, &far_cells, &far_nlc, &judge, &gwalk};
     for (DevBuf *b : all) b->release();
     n_alloc = 0;
 }

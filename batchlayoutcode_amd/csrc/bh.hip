@@ -754,7 +754,7 @@ __global__ __launch_bounds__(WG) void k_rep32(const double2 *__restrict__ qpos, 
     __shared__ uint64_t stk_mask[WG / 64][STACK];
     if (*status) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int k = (blockIdx.x * (WG / 64) + w) * 64 + lane;
+    const int k = (xcd_tile(blockIdx.x, gridDim.x) * (WG / 64) + w) * 64 + lane;
     const bool valid = k < n;
     const double2 p = qpos[valid ? k : n - 1];
     const double maxmin = frame[2];

@@ -29,6 +29,16 @@ __device__ __forceinline__ double rsqrt2(double x) {
     return y;
 }
 
+// The tile of workgroup b of nb.  The dispatcher deals workgroups to the 8 XCDs in turn
+// (workgroup b to XCD b % 8), so tile = start(b % 8) + b / 8 gives each XCD one contiguous
+// eighth of the tiles, and the workgroups in flight on an XCD take neighbouring tiles, whose
+// data (for Morton-ordered queries: the same subtrees) then shares that XCD's L2.  A bijection
+// of [0, nb) whatever the placement, so results do not depend on it.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int x = b & 7, r = b >> 3, q = nb >> 3, rem = nb & 7;
+    return x * q + min(x, rem) + r;
+}
+
 // Stores for another workgroup with agent-scope stores, which the XCD's L2 writes through to
 // memory: once they have completed (s_waitcnt), a relaxed flag or counter published after them
 // needs no L2 write-back, and a reader that acquired after seeing it reads the new values.
@@ -113,6 +123,7 @@ int wide_walk_prepare(bool has_w);
 // floats).  The other modes need the cell summaries of that tree first (far_cells_of).  ncells
 // (device, nullable) is incremented by the number of cells the traversal accepts.
 enum FarMode : int { FAR_MONO = 1, FAR_EXACT = 2, FAR_LEARNED = 3, FAR_QUAD = 4 };
+// Once per tree build: the climb consumes the arrival counters (BHNode.cnt) the build zeroed.
 int far_cells_of(BHWork &w, int n, hipStream_t s);
 // judged (FAR_LEARNED): the traversal follows the verdict of the last far_judge (the model, or
 // the analytic second-order correction when the model lost)

@@ -42,7 +42,7 @@ struct BHWork {
     DevBuf scan_tmp;
     DevBuf rep;                              // repulsion result per vertex (double2)
     DevBuf misc;                             // counters / status words
-    DevBuf far_cells, far_arrive;            // physically consistent far field: cell summaries
+    DevBuf far_cells, far_nlc;            // physically consistent far field: cell summaries
     DevBuf judge;                            // learned far field: the judge's sample, partials, flag
     DevBuf gwalk;                            // multi-workgroup walk: partials, barrier, error flag
     int gwalk_used = 0;                      // the last run launched it

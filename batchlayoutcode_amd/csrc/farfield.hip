@@ -11,7 +11,7 @@
 // b = -2 S_uw with S the cell's normalised second moments, from 1/conj(w + delta) expanded
 // in delta) or the output of the learned model of mlp.h scaled by (side/rho)^2.
 //
-//   k_far_small / k_far_top  count, mean, central second moments and first Morton rank of
+//   k_far_direct / k_far_top  count, mean, central second moments and first Morton rank of
 //                  every cell, bottom-up over the tree of bh.hip
 //   k_far_trav     traversal, one wave per 64 Morton-consecutive queries with a shared stack
 //                  of (node, lane mask); a cell is accepted when side^2 < theta^2 |c - v|^2.
@@ -44,10 +44,9 @@ struct FarCell {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Combines the non-empty quadrants of node v (children first): mean by counts, second
+// Combines the non-empty quadrants ch of a node (children first): mean by counts, second
 // moments by the parallel-axis rule, in quadrant order.
-__device__ FarCell far_summary(const BHNode *nodes, const double2 *spos, const FarCell *cells, int v) {
-    const int4 ch = nodes[v].ch;
+__device__ FarCell far_summary(int4 ch, const double2 *spos, const FarCell *cells) {
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
     double m = 0.0, sx = 0.0, sy = 0.0;
     int lo = 0x7fffffff;
@@ -110,21 +109,34 @@ __device__ __forceinline__ void store_cell_through(FarCell *cells, int v, const 
 
 // The summaries in two launches.  A node of at most FAR_SMALL points is summarised straight
 // from its points, Morton ranks [lo, lo + m), in two passes (the mean, then the central second
-// moments): one thread per node, no dependence between nodes.  The larger nodes climb, from
-// those without large children: a node's summary is stored with agent-scope stores (written
-// through to memory) and, once they have completed, a relaxed counter at the parent counts the
-// arrival; the last child to arrive acquires (invalidating its own L2's stale lines) and forms
-// the parent from its children by far_summary.  An acq_rel counter would write the whole L2
-// back at every step.
+// moments): one thread per node, no dependence between nodes; the same launch counts each larger
+// node's children of more than FAR_SMALL points (nlc).  The larger nodes climb, from those
+// without large children: a node's summary is stored with agent-scope stores (written through
+// to memory) and, once they have completed, a relaxed counter at the parent (BHNode.cnt, zeroed
+// by the tree build) counts the arrival; the last child to arrive acquires (invalidating its own
+// L2's stale lines) and forms the parent from its children by far_summary.  An acq_rel counter
+// would write the whole L2 back at every step.  What does not change in the launch (the
+// parent, its children and its nlc) is loaded before the hand-off, so a step waits for the
+// children's summaries, the stores and the counter only (k_far_top 145 -> 137 us on the
+// 1.66M-vertex Delaunay graph's layout; the climb's nine levels cost about 15 us each).
 constexpr int FAR_SMALL = 32;
 
-__global__ __launch_bounds__(WG) void k_far_direct(const double2 *__restrict__ spos, const int *__restrict__ npts,
-                                                   const int *__restrict__ node_lo, const int *__restrict__ misc,
-                                                   FarCell *__restrict__ cells) {
+__global__ __launch_bounds__(WG) void k_far_direct(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+                                                   const int *__restrict__ npts, const int *__restrict__ node_lo,
+                                                   const int *__restrict__ misc, FarCell *__restrict__ cells,
+                                                   int *__restrict__ nlc) {
     const int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
     const int m = npts[v];
-    if (m > FAR_SMALL) return;
+    if (m > FAR_SMALL) {
+        const int4 ch = nodes[v].ch;
+        const int c[4] = {ch.x, ch.y, ch.z, ch.w};
+        int k = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && npts[c[q]] > FAR_SMALL) ? 1 : 0;
+        nlc[v] = k;
+        return;
+    }
     const int lo = node_lo[v];
     double sx = 0.0, sy = 0.0;
     for (int k = 0; k < m; ++k) {
@@ -151,30 +163,29 @@ __global__ __launch_bounds__(WG) void k_far_direct(const double2 *__restrict__ s
     cells[v] = out;
 }
 
-__device__ __forceinline__ int far_large_children(const BHNode *nodes, const int *npts, int v) {
-    const int4 ch = nodes[v].ch;
-    const int c[4] = {ch.x, ch.y, ch.z, ch.w};
-    int k = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && npts[c[q]] > FAR_SMALL) ? 1 : 0;
-    return k;
-}
-
-__global__ __launch_bounds__(WG) void k_far_top(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
+__global__ __launch_bounds__(WG) void k_far_top(BHNode *nodes, const double2 *__restrict__ spos,
                                                 const int *__restrict__ npts, const int *__restrict__ misc,
-                                                FarCell *__restrict__ cells, int *__restrict__ arrive) {
+                                                const int *__restrict__ nlc, FarCell *__restrict__ cells) {
     int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
-    if (npts[v] <= FAR_SMALL || far_large_children(nodes, npts, v) != 0) return;
+    if (npts[v] <= FAR_SMALL || nlc[v] != 0) return;
+    int4 ch = nodes[v].ch;
     for (;;) {
-        store_cell_through(cells, v, far_summary(nodes, spos, cells, v));
         const int u = nodes[v].parent;
+        int need = 0;
+        int4 chu = make_int4(-1, -1, -1, -1);
+        if (u >= 0) {
+            need = nlc[u];
+            chu = nodes[u].ch;
+        }
+        store_cell_through(cells, v, far_summary(ch, spos, cells));
         if (u < 0) break;
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const int old = __hip_atomic_fetch_add(&arrive[u], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old != far_large_children(nodes, npts, u) - 1) break;
+        const int old = __hip_atomic_fetch_add(&nodes[u].cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old != need - 1) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         v = u;
+        ch = chu;
     }
 }
 
@@ -293,7 +304,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         __syncthreads();
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int k = (blockIdx.x * (WG / 64) + w) * 64 + lane;
+    const int k = (xcd_tile(blockIdx.x, gridDim.x) * (WG / 64) + w) * 64 + lane;
     const bool valid = k < nq;
     const double2 p = qpos[valid ? k : nq - 1];
     const double maxmin = frame[2], th2 = theta * theta;
@@ -647,7 +658,7 @@ __global__ __launch_bounds__(JUDGE_S) void k_judge_decide(int S, JudgeBuf *J) {
 }  // namespace
 
 static int ensure_far(BHWork &w, int n) {
-    if (w.far_cells.ensure(sizeof(FarCell) * (size_t)std::max(n, 1)) || w.far_arrive.ensure(4 * (size_t)std::max(n, 1)))
+    if (w.far_cells.ensure(sizeof(FarCell) * (size_t)std::max(n, 1)) || w.far_nlc.ensure(4 * (size_t)std::max(n, 1)))
         return BL_E_NOMEM;
     return BL_OK;
 }
@@ -657,12 +668,13 @@ int far_cells_of(BHWork &w, int n, hipStream_t s) {
     int rc = ensure_far(w, n);
     if (rc) return rc;
     if (n >= 2) {
-        BL_HIP(hipMemsetAsync(w.far_arrive.p, 0, 4 * (size_t)n, s));
-        hipLaunchKernelGGL(k_far_direct, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.spos.as<double2>(),
-                           w.node_cnt.as<int>(), w.node_lo.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>());
+        // the arrival counters (BHNode.cnt) were zeroed by the tree build, which forms no centroids
+        hipLaunchKernelGGL(k_far_direct, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
+                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.node_lo.as<int>(), w.misc.as<int>(),
+                           w.far_cells.as<FarCell>(), w.far_nlc.as<int>());
         hipLaunchKernelGGL(k_far_top, dim3((n + WG - 1) / WG), dim3(WG), 0, s, w.node_child.as<BHNode>(),
-                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_cells.as<FarCell>(),
-                           w.far_arrive.as<int>());
+                           w.spos.as<double2>(), w.node_cnt.as<int>(), w.misc.as<int>(), w.far_nlc.as<int>(),
+                           w.far_cells.as<FarCell>());
         BL_HIP(hipGetLastError());
     }
     return BL_OK;
