@@ -528,7 +528,7 @@ __device__ __forceinline__ int rep_of_range(const uint64_t *keys, int s, int e) 
 // first quadrant of the cell where k_i and k_i+1 split
 __global__ __launch_bounds__(WG) void k_tree_bounds(const uint64_t *__restrict__ keys, int n, int *__restrict__ isrep,
                                                     int *__restrict__ status) {
-    const int i = blockIdx.x * WG + threadIdx.x;
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * WG + threadIdx.x;
     if (i >= n - 1) return;
     const uint64_t a = keys[i], b = keys[i + 1];
     if (a == b) {
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(WG) void k_tree_nodes(const uint64_t *__restrict__ 
                                                    BHNode *__restrict__ nodes, int *__restrict__ npts,
                                                    int *__restrict__ node_lo, int *__restrict__ misc,
                                                    const int *__restrict__ status) {
-    const int i = blockIdx.x * WG + threadIdx.x;
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * WG + threadIdx.x;
     if (*status) return;
     if (i >= n - 1 || !isrep[i]) return;
     const int v = nodeid[i];

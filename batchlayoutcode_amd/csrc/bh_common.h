@@ -18,13 +18,14 @@ struct BHNode {
 };
 
 // 1/sqrt(x) for x > 0: the hardware estimate (v_rsq_f64, relative error about 2^-25) refined by
-// two Newton steps y <- y (3/2 - (x/2) y^2), within a few ulp of the correctly rounded value;
-// nine dependent operations where an IEEE square root and a division take about twenty-five.
+// one Newton step y <- y (3/2 - (x/2) y^2), within about 20 ulp of the correctly rounded value
+// (a second step, a few ulp, made no difference to the walks on the large graphs and cost the
+// one-batch-a-step walk 10%); five dependent operations where an IEEE square root and a
+// division take about twenty-five.
 // x = 0 gives NaN (0 * inf), as the reference's 0 / 0 does.
 __device__ __forceinline__ double rsqrt2(double x) {
     const double h = 0.5 * x;
     double y = __builtin_amdgcn_rsq(x);
-    y = y * fma(-h * y, y, 1.5);
     y = y * fma(-h * y, y, 1.5);
     return y;
 }

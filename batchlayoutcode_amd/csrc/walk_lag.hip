@@ -52,7 +52,7 @@
 //     every group it publishes the walker's progress to the helpers (groups whose stores
 //     completed: the computing waves wait for the stores of three groups back).
 // The computing waves' arithmetic is FP64 throughout; |d| and the unit vector F/|F| come from
-// the hardware reciprocal square root refined by two Newton steps (a few ulp from the IEEE
+// the hardware reciprocal square root refined by one Newton step (about 20 ulp from the IEEE
 // square root and division the row walk uses).
 // Barriers are raw s_barrier with explicit LDS waits (a workgroup barrier would also wait for the
 // loads in flight).  Helpers stop at a deadline or when the walker has finished, so the launch

@@ -37,8 +37,8 @@
 // What one workgroup writes for another (moved positions, pre_i) is stored through to memory
 // (agent-scope stores), so a relaxed flag or count store after the stores have completed
 // publishes them, and the reader acquires after seeing it.  The arithmetic is FP64: the
-// attraction terms and the move use the hardware reciprocal square root refined by two Newton
-// steps (attr_add, rsqrt2).
+// attraction terms and the move use the hardware reciprocal square root refined by one Newton
+// step (attr_add, rsqrt2).
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
