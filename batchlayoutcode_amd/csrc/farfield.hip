@@ -305,7 +305,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int k = (xcd_tile(blockIdx.x, gridDim.x) * (WG / 64) + w) * 64 + lane;
-    const bool valid = k < nq;
+    const bool valid = k < nq && qid[k] >= 0;  // a negative id: an idle lane (the judge's sample)
     const double2 p = qpos[valid ? k : nq - 1];
     const double maxmin = frame[2], th2 = theta * theta;
     double fx = 0.0, fy = 0.0;
@@ -581,13 +581,15 @@ constexpr int JUDGE_P = 1024;  // j chunks of the exact sum (four workgroups per
 // layout of BHWork::judge
 struct JudgeBuf {
     double2 q[JUDGE_S];            // sample positions
-    // the traversals' queries: sample vertex k in all 64 lanes of wave k (output slot k).  The
-    // sample is spread over the layout, so a wave of 64 different sample vertices would walk
-    // the union of 64 unrelated paths one dependent node load at a time (about 5 ms per
-    // traversal on 1.7M vertices); one vertex per wave walks one path, the waves in parallel.
+    // the traversals' queries: sample vertex k in lane 0 of wave k (output slot k), the other
+    // lanes idle (id -1).  The sample is spread over the layout, so a wave of 64 different
+    // sample vertices would walk the union of 64 unrelated paths one dependent node load at a
+    // time (about 5 ms per traversal on 1.7M vertices); one vertex per wave walks one path, the
+    // waves in parallel.  With the vertex in all 64 lanes the learned traversal queued every
+    // accepted cell 64 times (0.9 ms against 0.06 ms for the analytic one).
     double2 qw[JUDGE_S * 64];
     int qwid[JUDGE_S * 64];
-    double2 part[JUDGE_S * JUDGE_P];
+    double2 part[JUDGE_P * JUDGE_S];  // chunk-major: part[c * JUDGE_S + k]
     double2 field[2 * JUDGE_S];    // learned, then analytic, at the sample
     int use_model;                 // the verdict read by the traversals
     int model_iters;               // traversals that used the model
@@ -603,11 +605,11 @@ __global__ __launch_bounds__(JUDGE_S) void k_judge_sample(const double2 *__restr
     J->q[k] = q;
     for (int l = 0; l < 64; ++l) {
         J->qw[64 * k + l] = q;
-        J->qwid[64 * k + l] = k;
+        J->qwid[64 * k + l] = l == 0 ? k : -1;
     }
 }
 
-// part[k * P + c] = sum over points of chunk c of -(x_j - q_k)/|x_j - q_k|^2 (FP64-grade terms)
+// part[c * S + k] = sum over points of chunk c of -(x_j - q_k)/|x_j - q_k|^2 (FP64-grade terms)
 __global__ __launch_bounds__(JUDGE_S) void k_judge_exact(const double2 *__restrict__ pts, int n, int S,
                                                           JudgeBuf *__restrict__ J) {
     typedef const __attribute__((address_space(4))) double cdbl;
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(JUDGE_S) void k_judge_exact(const double2 *__restri
         for (int u = 0; u < 8; ++u) pair_rep<true>(cp[2 * (j + u)], cp[2 * (j + u) + 1], q.x, q.y, ax, ay);
     }
     for (; j < j1; ++j) pair_rep<true>(cp[2 * j], cp[2 * j + 1], q.x, q.y, ax, ay);
-    if (k < S) J->part[k * JUDGE_P + c] = make_double2(ax, ay);
+    if (k < S) J->part[c * JUDGE_S + k] = make_double2(ax, ay);
 }
 
 // Medians by rank (ties broken by index) and the verdict.
@@ -632,9 +634,9 @@ __global__ __launch_bounds__(JUDGE_S) void k_judge_decide(int S, JudgeBuf *J) {
     const int k = threadIdx.x;
     if (k < S) {
         double ex = 0.0, ey = 0.0;
-        for (int c = 0; c < JUDGE_P; ++c) {
-            ex -= J->part[k * JUDGE_P + c].x;
-            ey -= J->part[k * JUDGE_P + c].y;
+        for (int c = 0; c < JUDGE_P; ++c) {  // chunk order; the threads read consecutive entries
+            ex -= J->part[c * JUDGE_S + k].x;
+            ey -= J->part[c * JUDGE_S + k].y;
         }
         const double m = sqrt(ex * ex + ey * ey);
         for (int t = 0; t < 2; ++t) {
