@@ -118,8 +118,9 @@ __device__ __forceinline__ void store_cell_through(FarCell *cells, int v, const 
 // would write the whole L2 back at every step.  What does not change in the launch (the
 // parent, its children and its nlc) is loaded before the hand-off, so a step waits for the
 // children's summaries, the stores and the counter only (k_far_top 145 -> 137 us on the
-// 1.66M-vertex Delaunay graph's layout; the climb's nine levels cost about 15 us each).
-constexpr int FAR_SMALL = 32;
+// 1.66M-vertex Delaunay graph's layout at FAR_SMALL 32; the climb's nine levels cost about
+// 15 us each).
+constexpr int FAR_SMALL = 64;  // Delaunay graph, per iteration: 16: 251 us, 32: 204, 64: 179, 128: 191
 
 __global__ __launch_bounds__(WG) void k_far_direct(const BHNode *__restrict__ nodes, const double2 *__restrict__ spos,
                                                    const int *__restrict__ npts, const int *__restrict__ node_lo,
