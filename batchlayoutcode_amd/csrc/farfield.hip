@@ -35,12 +35,18 @@ constexpr int WG = 256;
 constexpr int STACK = 128;
 constexpr int QCAP = 128;  // queued cells per wave (a step adds at most 64)
 
-struct FarCell {
+// One 64-byte record per node, 64-byte aligned, so that a traversal step reads one cache line:
+// the summary, and the node's level and children copied from the tree (BHNode.depth, .ch).  A
+// summary and a BHNode read together took three lines a step on average.  The node's points
+// are Morton ranks [node_lo[v], node_lo[v] + m).
+struct __attribute__((aligned(64))) FarCell {
     double cx, cy;          // mean of the cell's points
     double mxx, mxy, myy;   // central second moments, sums over the points
     int m;                  // number of points
-    int lo;                 // first Morton rank; the points are ranks [lo, lo + m)
+    int depth;              // base-4 level of the cell: side = MAXMIN * 2^-depth
+    int4 ch;                // children: >= 0 node, -1 empty, <= -2 leaf at Morton rank (-c-2)
 };
+static_assert(sizeof(FarCell) == 64, "one cache line per node");
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -49,7 +55,6 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 __device__ FarCell far_summary(int4 ch, const double2 *spos, const FarCell *cells) {
     const int c[4] = {ch.x, ch.y, ch.z, ch.w};
     double m = 0.0, sx = 0.0, sy = 0.0;
-    int lo = 0x7fffffff;
     double km[4], kx[4], ky[4], kxx[4], kxy[4], kyy[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -63,13 +68,11 @@ __device__ FarCell far_summary(int4 ch, const double2 *spos, const FarCell *cell
             kxx[q] = k.mxx;
             kxy[q] = k.mxy;
             kyy[q] = k.myy;
-            lo = min(lo, k.lo);
         } else if (c[q] <= -2) {
             const double2 p = spos[-c[q] - 2];
             km[q] = 1.0;
             kx[q] = p.x;
             ky[q] = p.y;
-            lo = min(lo, -c[q] - 2);
         }
         m += km[q];
         sx += km[q] * kx[q];
@@ -90,27 +93,26 @@ __device__ FarCell far_summary(int4 ch, const double2 *spos, const FarCell *cell
     out.mxy = b;
     out.myy = d;
     out.m = (int)m;
-    out.lo = lo;
     return out;
 }
 
 // Stores cell v with agent-scope stores, which the XCD's L2 writes through to memory.
+// The summary fields (not depth and ch, written by k_far_direct).
 __device__ __forceinline__ void store_cell_through(FarCell *cells, int v, const FarCell &c) {
-    static_assert(sizeof(FarCell) == 48, "five doubles, then m and lo");
-    double *d = reinterpret_cast<double *>(cells + v);
-    __hip_atomic_store(d + 0, c.cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(d + 1, c.cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(d + 2, c.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(d + 3, c.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(d + 4, c.myy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long ml = (unsigned long long)(unsigned)c.m | ((unsigned long long)(unsigned)c.lo << 32);
-    __hip_atomic_store(reinterpret_cast<unsigned long long *>(d + 5), ml, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    FarCell *d = cells + v;
+    __hip_atomic_store(&d->cx, c.cx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&d->cy, c.cy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&d->mxx, c.mxx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&d->mxy, c.mxy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&d->myy, c.myy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&d->m, c.m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The summaries in two launches.  A node of at most FAR_SMALL points is summarised straight
 // from its points, Morton ranks [lo, lo + m), in two passes (the mean, then the central second
-// moments): one thread per node, no dependence between nodes; the same launch counts each larger
-// node's children of more than FAR_SMALL points (nlc).  The larger nodes climb, from those
+// moments): one thread per node, no dependence between nodes; the same launch copies every
+// node's level and children into its record and counts each larger node's children of more
+// than FAR_SMALL points (nlc).  The larger nodes climb, from those
 // without large children: a node's summary is stored with agent-scope stores (written through
 // to memory) and, once they have completed, a relaxed counter at the parent (BHNode.cnt, zeroed
 // by the tree build) counts the arrival; the last child to arrive acquires (invalidating its own
@@ -129,13 +131,16 @@ __global__ __launch_bounds__(WG) void k_far_direct(const BHNode *__restrict__ no
     const int v = blockIdx.x * WG + threadIdx.x;
     if (misc[0] || v >= misc[2]) return;
     const int m = npts[v];
+    const int4 ch = nodes[v].ch;
+    const int depth = nodes[v].depth;
     if (m > FAR_SMALL) {
-        const int4 ch = nodes[v].ch;
         const int c[4] = {ch.x, ch.y, ch.z, ch.w};
         int k = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && npts[c[q]] > FAR_SMALL) ? 1 : 0;
         nlc[v] = k;
+        cells[v].depth = depth;
+        cells[v].ch = ch;
         return;
     }
     const int lo = node_lo[v];
@@ -160,7 +165,8 @@ __global__ __launch_bounds__(WG) void k_far_direct(const BHNode *__restrict__ no
     out.mxy = b;
     out.myy = d;
     out.m = m;
-    out.lo = lo;
+    out.depth = depth;
+    out.ch = ch;
     cells[v] = out;
 }
 
@@ -337,11 +343,10 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         const uint64_t m = stk_mask[w][sp];
         __builtin_amdgcn_wave_barrier();
         const bool act = (m >> lane) & 1ull;
-        const FarCell cl = cells[v];
-        const BHNode nd = nodes[v];
+        const FarCell cl = cells[v];  // one line: the summary, the level and the children
         const double dx = cl.cx - p.x, dy = cl.cy - p.y;
         const double r2 = dx * dx + dy * dy;
-        const double D = ldexp(maxmin, -nd.depth);
+        const double D = ldexp(maxmin, -cl.depth);
         const bool acc = D * D < th2 * r2;
         if (ncells) accepted += (unsigned)__popcll(__ballot(act && acc));
         if (act && acc) {
@@ -396,7 +401,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
         }
         const uint64_t rej = __ballot(act && !acc);
         if (rej) {
-            const int c[4] = {nd.ch.x, nd.ch.y, nd.ch.z, nd.ch.w};
+            const int c[4] = {cl.ch.x, cl.ch.y, cl.ch.z, cl.ch.w};
 #pragma unroll
             for (int q = 3; q >= 0; --q) {
                 if (c[q] >= 0) {
@@ -430,7 +435,7 @@ __global__ __launch_bounds__(WG) void k_far_trav(const double2 *__restrict__ qpo
 // points in FP64.  Rows k*cap .. k*cap + count[k] - 1, in traversal order.
 __global__ __launch_bounds__(WG) void k_far_samples(const double2 *__restrict__ qpos, int nq,
                                                     const double2 *__restrict__ spos,
-                                                    const BHNode *__restrict__ nodes,
+                                                    const BHNode *__restrict__ nodes, const int *__restrict__ node_lo,
                                                     const FarCell *__restrict__ cells, const int *__restrict__ misc,
                                                     const double *__restrict__ frame, double theta, int cap,
                                                     float *__restrict__ feat, float *__restrict__ targ,
@@ -469,7 +474,7 @@ __global__ __launch_bounds__(WG) void k_far_samples(const double2 *__restrict__ 
             const bool take = act && acc && cnt < cap;
             if (__ballot(take)) {
                 double ex = 0.0, ey = 0.0;
-                const int lo = uni(cl.lo), hi = lo + uni(cl.m);
+                const int lo = uni(node_lo[v]), hi = lo + uni(cl.m);
                 for (int j = lo; j < hi; ++j) {  // same points for every lane: broadcast loads
                     const double2 q = spos[j];
                     const double qx = q.x - p.x, qy = q.y - p.y;
@@ -797,8 +802,9 @@ int far_samples_host(int32_t n, const double *xy, double theta, int32_t nq, cons
     BL_HIP(hipMemsetAsync(G.c.p, 0, 4 * (size_t)nq, G.s));
     const int g64 = ((nq + 63) / 64 + WG / 64 - 1) / (WG / 64);
     hipLaunchKernelGGL(k_far_samples, dim3(g64), dim3(WG), 0, G.s, G.qs.as<double2>(), nq, G.w.spos.as<double2>(),
-                       G.w.node_child.as<BHNode>(), G.w.far_cells.as<FarCell>(), G.w.misc.as<int>(),
-                       G.w.frame.as<double>(), theta, cap, G.f.as<float>(), G.t.as<float>(), G.c.as<int>());
+                       G.w.node_child.as<BHNode>(), G.w.node_lo.as<int>(), G.w.far_cells.as<FarCell>(),
+                       G.w.misc.as<int>(), G.w.frame.as<double>(), theta, cap, G.f.as<float>(), G.t.as<float>(),
+                       G.c.as<int>());
     BL_HIP(hipGetLastError());
     rc = G.status();
     if (rc) return rc;
