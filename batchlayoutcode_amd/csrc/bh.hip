@@ -8,8 +8,9 @@
 // Per iteration, from the positions at the start of the iteration (:1041-1047):
 //   1. k_frame_*   min x, min y, MAXMIN = max(dy, dx)                       (FP64, exact)
 //   2. k_keys      64-bit key = interleave(yscale, xscale), y bit above x bit
-//   3. radix sort  LSD, 8 passes of 8 bits over (key, vertex id), stable: per-tile digit
-//                  counts, a scan per digit, a scatter staged through LDS
+//   3. radix sort  LSD, 6 passes of 8 bits over (key, vertex id), stable: per-tile digit
+//                  counts, a scan per digit, a scatter staged through LDS; the keys' low 16
+//                  bits order only runs of keys equal above them, sorted in place (k_sort_ties)
 //   4. gather      positions in Morton order (the last scatter pass writes them)
 //   5. tree        node = cell where two adjacent keys first differ (base-4 digit d of
 //                  k_i ^ k_i+1); its range, quadrants and children come from galloping and
@@ -462,6 +463,38 @@ __global__ __launch_bounds__(WG) void k_sort_scatter(const uint64_t *__restrict_
             vout[g] = id;
             if (GATHER) spos[g] = pos[id];
         }
+    }
+}
+
+// The radix passes order the keys by their top 64 - SORT_SKIP bits only: keys equal in those
+// bits belong to points within 2^-24 of the frame's side of each other on both axes, rare and
+// in short runs.  The passes are stable, so a run is in vertex-id order; thread i, at the start
+// of a run, puts it in full-key order by a stable insertion sort, moving the ids and positions
+// with the keys: the result is the full 64-bit sort.  Other threads read the run's keys while
+// it is being sorted only to compare their top bits, which every key of the run shares.
+constexpr int SORT_SKIP = 16;
+__global__ __launch_bounds__(WG) void k_sort_ties(uint64_t *__restrict__ keys, int *__restrict__ ids,
+                                                  double2 *__restrict__ spos, int n) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i + 1 >= n) return;
+    const uint64_t h = keys[i] >> SORT_SKIP;
+    if ((keys[i + 1] >> SORT_SKIP) != h) return;
+    if (i > 0 && (keys[i - 1] >> SORT_SKIP) == h) return;  // inside a run, not its first key
+    int e = i + 2;
+    while (e < n && (keys[e] >> SORT_SKIP) == h) ++e;
+    for (int a = i + 1; a < e; ++a) {
+        const uint64_t k = keys[a];
+        const int id = ids[a];
+        const double2 p = spos[a];
+        int b = a - 1;
+        for (; b >= i && keys[b] > k; --b) {
+            keys[b + 1] = keys[b];
+            ids[b + 1] = ids[b];
+            spos[b + 1] = spos[b];
+        }
+        keys[b + 1] = k;
+        ids[b + 1] = id;
+        spos[b + 1] = p;
     }
 }
 
@@ -1988,7 +2021,7 @@ int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *
     int *hist = w.hist.as<int>();
     int *dtot = w.scan_tmp.as<int>();
     int *bsum = misc + 64;
-    for (int pass = 0; pass < 8; ++pass) {
+    for (int pass = SORT_SKIP / 8; pass < 8; ++pass) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(WG), 0, s, ka, n, shift, nblk, rounds, hist);
         hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(WG), 0, s, hist, nblk, dtot);
@@ -2001,7 +2034,10 @@ int build_tree(BHWork &w, const double2 *d_pos, int n, hipStream_t s, uint32_t *
         std::swap(ka, kb);
         std::swap(ia, ib);
     }
-    // after 8 passes the sorted data is back in keys_a / idx_a, the positions in spos
+    // after 6 passes the data is back in keys_a / idx_a, the positions in spos, in the order of
+    // the keys' top 48 bits; runs of keys equal in those bits are put in full-key order here
+    static_assert((8 - SORT_SKIP / 8) % 2 == 0, "an even number of passes ends in keys_a");
+    if (n >= 2) hipLaunchKernelGGL(k_sort_ties, dim3(g1), dim3(WG), 0, s, ka, ia, spos, n);
     if (n >= 2) {
         int *isrep = w.isrep.as<int>(), *nodeid = w.nodeid.as<int>();
         BHNode *nodes = w.node_child.as<BHNode>();

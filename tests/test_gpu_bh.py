@@ -83,6 +83,35 @@ def test_sort_tile_and_scan_boundaries(bl, oracle, n):
     assert np.array_equal(got, oracle.bh_forces(pts, 1.2, queries=q))
 
 
+def test_sort_runs_equal_in_the_top_48_key_bits(bl, oracle):
+    """The radix passes order the Morton keys by their top 48 bits (24 per axis); keys equal
+    there, points within 2^-24 of the frame's side on both axes, are put in full-key order
+    afterwards (k_sort_ties).  Pairs and triples of points 2^-28 apart in random directions,
+    the points in shuffled order so that a run's ids are not already in key order: the
+    permutation matches the oracle's full sort, and the FP64 forces stay bit-exact."""
+    rng = np.random.default_rng(11)
+    base = rng.random((20000, 2))
+
+    def off(k):  # at least 2^-30 per axis: the 32-bit scales differ (distinct keys)
+        return rng.choice([-1.0, 1.0], (k, 2)) * rng.uniform(0.25, 1.0, (k, 2)) * 2.0**-28
+
+    pts = np.concatenate([base, base[:3000] + off(3000), base[:500] + off(500)])
+    pts = pts[rng.permutation(len(pts))]
+    xs, ys, _ = oracle.morton(pts)
+    top = (ys.astype(np.uint64) >> np.uint64(8)) << np.uint64(24) | (xs.astype(np.uint64) >> np.uint64(8))
+    _, cnt = np.unique(top, return_counts=True)
+    full = ys.astype(np.uint64) << np.uint64(32) | xs.astype(np.uint64)
+    assert len(np.unique(full)) == len(pts)  # no duplicate points
+    print(f"[case] sort ties: {int((cnt > 1).sum())} runs, longest {int(cnt.max())}", flush=True)
+    assert (cnt > 1).sum() >= 2000 and cnt.max() >= 3
+    t = bl.bh_tree(pts)
+    ref = oracle.bh_tree(pts)
+    assert np.array_equal(t["perm"], ref["perm"])
+    q = pts[rng.choice(len(pts), 64, replace=False)]
+    got = bl.bh_forces(pts, 1.2, queries=q, precision=bl.PREC_FP64)
+    assert np.array_equal(got, oracle.bh_forces(pts, 1.2, queries=q))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5])
 def test_tiny_point_sets(bl, oracle, n):
     pts = np.random.default_rng(n).random((n, 2))
